@@ -1,0 +1,169 @@
+/*
+ * tbh.h -- C ABI of the MI355X tropical contraction backend (libtbh.so).
+ *
+ * Drop-in for the contraction step of TensorBranching.jl: what the Julia
+ * side binds with ccall in place of GenericTensorNetworks.solve(...,
+ * SizeMax(); T, usecuda) over OMEinsum + TropicalGEMM/CuTropicalGEMM.
+ * Plain pointers and sizes only.  Every call returns 0 on success or a
+ * negative TBH_E* code; tbh_last_error() (thread-local) says why.
+ *
+ * Data conventions
+ *   - A slice is what SlicedBranch holds (src/types.jl:85-103): the graph's
+ *     vertex count and weights, the CompressedEinsum leaf label lists `ixs`
+ *     and binary ContractionTree `ct` (src/types.jl:51-58), and the fixed
+ *     weight r.  Labels are 0-based vertex ids.  Leaf content follows from
+ *     its rank, as GenericTensorNetworks builds IndependentSet tensors for
+ *     SizeMax: [v] -> [0, w_v]; [u,v] -> [0 0; 0 -inf].
+ *   - tree: the internal nodes in post-order; node j has children
+ *     tree[2j], tree[2j+1]; a child id c < nleaves is leaf c, otherwise the
+ *     internal node c - nleaves (< j).  The last node is the root.
+ *   - Intermediate labels follow parse_eincode: a label of a subtree is kept
+ *     iff it occurs outside it; the output labels of the network are empty.
+ *   - Tensors are column-major: label position i has stride 2^i.
+ *   - Tropical Float32: (+) = max, (x) = +, zero = -inf.  Tropical Int32:
+ *     zero = TBH_I32_NEG_INF (-2^30); every stored entry below -2^29 is
+ *     rewritten to it, so finite values must satisfy |x| < 2^28.
+ *   - Results are bit-exact: values are integers (or, for f32, any values:
+ *     max is exact and rounding is monotone), so any evaluation order of the
+ *     same tree gives identical bits.
+ */
+#ifndef TBH_H
+#define TBH_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TBH_I32_NEG_INF (-(1 << 30))
+
+typedef enum { TBH_F32 = 0, TBH_I32 = 1 } tbh_dtype;
+
+/* PLAIN: one slice at a time, one launch per step of the OMEinsum lowering
+ * (permute, GEMM, permute), synchronous per slice -- the per-slice GPU loop
+ * of contract_slices(..., usecuda=true).  BATCHED: all slices of a batch run
+ * wave by wave, one persistent launch per wave and kernel kind. */
+typedef enum { TBH_PLAIN = 0, TBH_BATCHED = 1 } tbh_mode;
+
+typedef enum { TBH_HOST = 0, TBH_DEVICE = 1 } tbh_ptr_kind;
+
+enum {
+  TBH_OK = 0,
+  TBH_EINVAL = -1,   /* bad argument or malformed slice */
+  TBH_ENOMEM = -2,   /* does not fit the pool */
+  TBH_EDEVICE = -3,  /* HIP runtime error */
+  TBH_EBUSY = -4,    /* serving queue full */
+  TBH_ENOTFOUND = -5 /* unknown ticket */
+};
+
+typedef struct tbh_ctx tbh_ctx;
+
+/* One SlicedBranch (src/types.jl:85-103). */
+typedef struct {
+  int32_t nv;             /* vertices; 0 -> the slice's result is r          */
+  const void* weights;    /* nv values of the dtype; NULL = UnitWeight       */
+  int32_t nleaves;        /* length(code.ixs)                                */
+  const int32_t* ix_ptr;  /* nleaves + 1 offsets into ix                     */
+  const int32_t* ix;      /* leaf labels (0-based vertex ids)                */
+  const int32_t* tree;    /* 2*(nleaves-1) child ids in post-order (code.ct) */
+  double r;               /* branch.r                                        */
+} tbh_slice;
+
+/* Context: one GPU, one device memory pool of pool_bytes (0 = 80% of the
+ * free memory), one stream.  Replaces the implicit CUDA.jl state used by
+ * solve(..., usecuda=true) (src/dynamic_ob.jl:32). */
+int tbh_create(int device, size_t pool_bytes, tbh_ctx** out);
+int tbh_destroy(tbh_ctx* ctx);
+const char* tbh_last_error(void);
+
+/* contract_slices(branches, T, usecuda) (src/dynamic_ob.jl:36-48): results[i]
+ * = solve_slice(slice i) + T(r_i), or T(r_i) when nv == 0.  results is a
+ * host array of n dtype values. */
+int tbh_contract_slices(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
+                        size_t n, void* results, tbh_mode mode);
+
+/* solve_slice(branch, T, usecuda) (src/dynamic_ob.jl:30-34): the contraction
+ * of one slice WITHOUT r (the tropical one, 0, for a slice with no leaves). */
+int tbh_solve_slice(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slice,
+                    void* result, tbh_mode mode);
+
+/* maximum(contract_slices(...)) (src/dynamic_ob.jl:24-27) in one call. */
+int tbh_contract_max(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
+                     size_t n, void* result, tbh_mode mode);
+
+/* Max-plus GEMM, the lower seam mul!(C, A, B) on (Cu)Matrix{Tropical{T}}
+ * that TropicalGEMM / CuTropicalGEMM supply (README.md:14-22):
+ *   C[:,:,t] = A[:,:,t] (x) B[:,:,t],  C[m,n] = max_k A[m,k] + B[k,n]
+ * column-major with leading dimensions and batch strides (in elements).
+ * ptr_kind TBH_HOST: host buffers (copied through the pool);
+ * TBH_DEVICE: device pointers, run on the context's stream, synchronous. */
+int tbh_maxplus_gemm(tbh_ctx* ctx, tbh_dtype dtype, int64_t M, int64_t N,
+                     int64_t K, int64_t batch, const void* A, int64_t lda,
+                     int64_t strideA, const void* B, int64_t ldb,
+                     int64_t strideB, void* C, int64_t ldc, int64_t strideC,
+                     tbh_ptr_kind ptr_kind);
+
+/* Complexity with every label of size 2 (complexity(branch),
+ * src/types.jl:115-121): tc, sc, rwc as log2. */
+int tbh_complexity(const tbh_slice* slice, double* tc, double* sc,
+                   double* rwc);
+
+/* contraction_peak_memory / contraction_all_memory (src/utils.jl:197-229),
+ * log2 element counts of the reference memory model. */
+int tbh_peak_memory(const tbh_slice* slice, double* log2_elems);
+int tbh_all_memory(const tbh_slice* slice, double* log2_elems);
+
+/* Device bytes the BATCHED executor reserves for this slice (its arena, the
+ * admission footprint used by the pool and the serving scheduler). */
+int tbh_slice_footprint(const tbh_slice* slice, tbh_dtype dtype,
+                        size_t* bytes);
+
+/* ---- serving: continuous batching of MIS requests -------------------- */
+/* A request is one MIS solve: maximum over its slices of contract_slices. */
+typedef struct {
+  tbh_dtype dtype;
+  const tbh_slice* slices;  /* copied at submit; caller may free afterwards */
+  size_t n;
+} tbh_request;
+
+typedef enum {
+  TBH_QUEUED = 0,
+  TBH_RUNNING = 1,
+  TBH_DONE = 2,
+  TBH_FAILED = 3
+} tbh_status;
+
+/* Start the scheduler thread; requests are admitted while the summed
+ * footprint of resident slices stays within the pool and at most
+ * max_resident_slices are resident. */
+int tbh_serve_start(tbh_ctx* ctx, size_t max_resident_slices);
+int tbh_serve_stop(tbh_ctx* ctx);
+/* Thread-safe (many producers). */
+int tbh_submit(tbh_ctx* ctx, const tbh_request* req, uint64_t* ticket);
+/* Non-blocking; result (one dtype value, the MIS) valid when DONE.  Times
+ * in seconds on the scheduler's monotonic clock. */
+int tbh_poll(tbh_ctx* ctx, uint64_t ticket, void* result, tbh_status* status,
+             double* t_submit, double* t_done);
+/* Blocks until the request is DONE or FAILED. */
+int tbh_wait(tbh_ctx* ctx, uint64_t ticket, void* result, tbh_status* status);
+/* Drops a queued (not yet admitted) request. */
+int tbh_cancel(tbh_ctx* ctx, uint64_t ticket);
+
+/* ---- introspection ---------------------------------------------------- */
+typedef struct {
+  uint64_t launches;        /* kernel launches issued                     */
+  uint64_t waves;           /* batched waves executed                     */
+  uint64_t ew_ops, gemm_ops, leaf_tensors;
+  double ew_ops_algo_bytes; /* algorithmic bytes of EW ops (|A|+|B|+|C|)   */
+  double gemm_ops_algo_ops; /* 2 * MACs of GEMM ops                        */
+  double total_algo_ops;    /* 2 * MACs over all ops                       */
+  size_t pool_bytes, pool_peak_bytes;
+} tbh_stats;
+int tbh_get_stats(tbh_ctx* ctx, tbh_stats* out);
+int tbh_reset_stats(tbh_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

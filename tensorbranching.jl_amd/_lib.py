@@ -1,0 +1,138 @@
+"""ctypes bindings of libtbh.so (include/tbh.h) and libtbh_workload.so
+(include/tbh_workload.h).
+
+The product path is libtbh.so only: loading fails loudly when it is missing,
+and nothing here computes a contraction on the CPU.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+TBH_F32, TBH_I32 = 0, 1
+TBH_PLAIN, TBH_BATCHED = 0, 1
+TBH_HOST, TBH_DEVICE = 0, 1
+TBH_I32_NEG_INF = -(1 << 30)
+
+STATUS = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+
+
+class TbhSlice(C.Structure):
+    _fields_ = [("nv", C.c_int32), ("weights", C.c_void_p), ("nleaves", C.c_int32),
+                ("ix_ptr", C.c_void_p), ("ix", C.c_void_p), ("tree", C.c_void_p),
+                ("r", C.c_double)]
+
+
+class TbhRequest(C.Structure):
+    _fields_ = [("dtype", C.c_int), ("slices", C.POINTER(TbhSlice)), ("n", C.c_size_t)]
+
+
+class TbhStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("waves", C.c_uint64), ("ew_ops", C.c_uint64),
+                ("gemm_ops", C.c_uint64), ("leaf_tensors", C.c_uint64),
+                ("ew_ops_algo_bytes", C.c_double), ("gemm_ops_algo_ops", C.c_double),
+                ("total_algo_ops", C.c_double), ("pool_bytes", C.c_size_t),
+                ("pool_peak_bytes", C.c_size_t)]
+
+
+class TbwSliceView(C.Structure):
+    _fields_ = [("nv", C.c_int32), ("ne", C.c_int32), ("edges", C.POINTER(C.c_int32)),
+                ("weights", C.POINTER(C.c_double)), ("nleaves", C.c_int32),
+                ("ix_ptr", C.POINTER(C.c_int32)), ("ix", C.POINTER(C.c_int32)),
+                ("tree", C.POINTER(C.c_int32)), ("r", C.c_double), ("sc", C.c_double),
+                ("tc", C.c_double)]
+
+
+# Every function the C headers declare, with its ctypes signature.
+TBH_SIGNATURES = {
+    "tbh_create": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "tbh_destroy": (C.c_int, [C.c_void_p]),
+    "tbh_last_error": (C.c_char_p, []),
+    "tbh_contract_slices": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(TbhSlice), C.c_size_t,
+                                      C.c_void_p, C.c_int]),
+    "tbh_solve_slice": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(TbhSlice), C.c_void_p, C.c_int]),
+    "tbh_contract_max": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(TbhSlice), C.c_size_t,
+                                   C.c_void_p, C.c_int]),
+    "tbh_maxplus_gemm": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                   C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64,
+                                   C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.c_int]),
+    "tbh_complexity": (C.c_int, [C.POINTER(TbhSlice), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "tbh_peak_memory": (C.c_int, [C.POINTER(TbhSlice), C.POINTER(C.c_double)]),
+    "tbh_all_memory": (C.c_int, [C.POINTER(TbhSlice), C.POINTER(C.c_double)]),
+    "tbh_slice_footprint": (C.c_int, [C.POINTER(TbhSlice), C.c_int, C.POINTER(C.c_size_t)]),
+    "tbh_serve_start": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "tbh_serve_stop": (C.c_int, [C.c_void_p]),
+    "tbh_submit": (C.c_int, [C.c_void_p, C.POINTER(TbhRequest), C.POINTER(C.c_uint64)]),
+    "tbh_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_int),
+                           C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "tbh_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_int)]),
+    "tbh_cancel": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "tbh_get_stats": (C.c_int, [C.c_void_p, C.POINTER(TbhStats)]),
+    "tbh_reset_stats": (C.c_int, [C.c_void_p]),
+}
+
+TBW_SIGNATURES = {
+    "tbw_last_error": (C.c_char_p, []),
+    "tbw_random_regular": (C.c_void_p, [C.c_int32, C.c_int32, C.c_uint64]),
+    "tbw_erdos_renyi": (C.c_void_p, [C.c_int32, C.c_double, C.c_uint64]),
+    "tbw_ksg": (C.c_void_p, [C.c_int32, C.c_int32, C.c_double, C.c_uint64]),
+    "tbw_graph_from_edges": (C.c_void_p, [C.c_int32, C.c_int32, C.c_void_p]),
+    "tbw_graph_nv": (C.c_int32, [C.c_void_p]),
+    "tbw_graph_ne": (C.c_int32, [C.c_void_p]),
+    "tbw_graph_edges": (C.POINTER(C.c_int32), [C.c_void_p]),
+    "tbw_graph_free": (None, [C.c_void_p]),
+    "tbw_elimination_order": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p]),
+    "tbw_slice": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint64,
+                               C.c_int32, C.c_int32]),
+    "tbw_whole": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64]),
+    "tbw_sliceset_count": (C.c_int64, [C.c_void_p]),
+    "tbw_sliceset_get": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TbwSliceView)]),
+    "tbw_sliceset_free": (None, [C.c_void_p]),
+}
+
+
+def _load(name, sigs):
+    path = os.path.join(LIBDIR, name)
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is not built: run tensorbranching.jl_amd/build.py (or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for fn, (res, args) in sigs.items():
+        f = getattr(lib, fn)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_tbh = None
+_tbw = None
+
+
+def tbh():
+    """libtbh.so (the HIP backend); raises if it is not built."""
+    global _tbh
+    if _tbh is None:
+        _tbh = _load("libtbh.so", TBH_SIGNATURES)
+    return _tbh
+
+
+def tbw():
+    """libtbh_workload.so (host producers of slices)."""
+    global _tbw
+    if _tbw is None:
+        _tbw = _load("libtbh_workload.so", TBW_SIGNATURES)
+    return _tbw
+
+
+class TbhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tbh error {code}: {msg}")
+        self.code = code
+
+
+def check(rc):
+    if rc != 0:
+        raise TbhError(rc, tbh().tbh_last_error().decode(errors="replace"))
+    return rc

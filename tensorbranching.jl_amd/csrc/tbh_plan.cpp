@@ -1,0 +1,601 @@
+// tbh_plan.cpp -- lowers a slice's binary contraction tree to EW / GEMM steps.
+//
+// Semantics follow the reference path: leaf tensors of IndependentSet under
+// SizeMax (GenericTensorNetworks generate_tensors), intermediate labels by
+// parse_eincode's rule (a label is kept iff it occurs outside the subtree;
+// uncompress, src/types.jl:75-79), and each node a tropical contraction that
+// max-reduces the labels it drops (OMEinsum's binary rule).  The two modes:
+//
+//  - plain (OMEinsum's lowering, the reference GPU path): per node, permute
+//    and reduce the inputs to (a,k,t) / (k,b,t), one batched max-plus GEMM to
+//    (a,b,t), permute to the node's label order; steps run one by one.
+//  - batched: each node is one EW step reading its inputs in whatever layout
+//    they have and writing its output in the layout its consumer wants, or a
+//    tiled GEMM step when the node is GEMM-shaped (its inputs are written in
+//    GEMM order by their producers); steps are levelled into waves (ALAP) so
+//    a wave of many slices is one launch per kernel kind.
+//
+// Default layout of a tensor: labels ordered by when they are eliminated,
+// latest first, so long-lived labels sit at the low (contiguous) positions of
+// every tensor and consecutive lanes stay coalesced along the spine.
+#include "tbh_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+namespace tbh {
+
+namespace {
+
+int fail(std::string* err, const std::string& m) {
+  if (err) *err = m;
+  return -1;
+}
+
+int validate(const SliceIn& s, std::string* err) {
+  if (s.nv < 0 || s.nleaves < 0) return fail(err, "negative nv or nleaves");
+  if (s.nleaves == 0) return 0;
+  if (!s.ix_ptr || !s.ix) return fail(err, "missing leaf labels");
+  if (s.nleaves > 1 && !s.tree) return fail(err, "missing tree");
+  if (s.ix_ptr[0] != 0) return fail(err, "ix_ptr[0] must be 0");
+  for (int32_t i = 0; i < s.nleaves; i++) {
+    int32_t n = s.ix_ptr[i + 1] - s.ix_ptr[i];
+    if (n < 1 || n > 2)
+      return fail(err, "leaf " + std::to_string(i) + " has rank " + std::to_string(n) +
+                           " (MIS leaves have rank 1 or 2)");
+    for (int32_t k = s.ix_ptr[i]; k < s.ix_ptr[i + 1]; k++)
+      if (s.ix[k] < 0 || s.ix[k] >= s.nv)
+        return fail(err, "leaf label " + std::to_string(s.ix[k]) + " out of range");
+    if (n == 2 && s.ix[s.ix_ptr[i]] == s.ix[s.ix_ptr[i] + 1])
+      return fail(err, "edge leaf with a repeated label");
+  }
+  if (s.nleaves > 1) {
+    int32_t tot = 2 * s.nleaves - 1;
+    std::vector<char> used(tot, 0);
+    for (int32_t j = 0; j + 1 < s.nleaves; j++)
+      for (int q = 0; q < 2; q++) {
+        int32_t c = s.tree[2 * j + q];
+        if (c < 0 || c >= tot || (c >= s.nleaves && c - s.nleaves >= j) || used[c])
+          return fail(err, "malformed tree at node " + std::to_string(j));
+        used[c] = 1;
+      }
+  }
+  return 0;
+}
+
+// labels of leaves then internal nodes, each in first-appearance order
+void compute_labels(const SliceIn& s, std::vector<std::vector<int32_t>>& lab) {
+  int32_t nl = s.nleaves, nn = std::max(0, nl - 1);
+  lab.assign(nl + nn, {});
+  std::vector<std::vector<int32_t>> cnt(nl + nn);
+  std::vector<int32_t> total(std::max(1, s.nv), 0), acc(std::max(1, s.nv), 0);
+  for (int32_t i = 0; i < nl; i++)
+    for (int32_t k = s.ix_ptr[i]; k < s.ix_ptr[i + 1]; k++) {
+      lab[i].push_back(s.ix[k]);
+      cnt[i].push_back(1);
+      total[s.ix[k]]++;
+    }
+  for (int32_t j = 0; j < nn; j++) {
+    int32_t a = s.tree[2 * j], b = s.tree[2 * j + 1], o = nl + j;
+    for (int32_t c : {a, b})
+      for (size_t k = 0; k < lab[c].size(); k++) acc[lab[c][k]] += cnt[c][k];
+    for (int32_t c : {a, b})
+      for (size_t k = 0; k < lab[c].size(); k++) {
+        int32_t l = lab[c][k];
+        if (acc[l] > 0) {
+          if (acc[l] < total[l]) {
+            lab[o].push_back(l);
+            cnt[o].push_back(acc[l]);
+          }
+          acc[l] = 0;
+        }
+      }
+  }
+}
+
+int pos_in(const std::vector<int32_t>& v, int32_t x) {
+  for (size_t i = 0; i < v.size(); i++)
+    if (v[i] == x) return (int)i;
+  return -1;
+}
+
+struct Builder {
+  const SliceIn& s;
+  const PlanOptions& opt;
+  SlicePlan& P;
+  std::vector<std::vector<int32_t>> layout;  // per tensor
+  std::string* err;
+
+  int add_tensor(const std::vector<int32_t>& lay) {
+    PTensor t;
+    t.rank = (int)lay.size();
+    t.bytes = (int64_t)4 << t.rank;
+    P.tensors.push_back(t);
+    layout.push_back(lay);
+    return (int)P.tensors.size() - 1;
+  }
+
+  // EW step C = max over dropped labels of A (+ B)
+  int ew(int a, int b, int c) {
+    const auto &LA = layout[a], &LC = layout[c];
+    static const std::vector<int32_t> none;
+    const auto& LB = b >= 0 ? layout[b] : none;
+    POp op;
+    op.kind = kEw;
+    op.a = a;
+    op.b = b;
+    op.c = c;
+    op.nP = (int)LC.size();
+    if (op.nP > kMaxRank) return fail(err, "tensor rank above 30");
+    std::memset(op.pa, -1, sizeof op.pa);
+    std::memset(op.pb, -1, sizeof op.pb);
+    std::memset(op.ra, -1, sizeof op.ra);
+    std::memset(op.rb, -1, sizeof op.rb);
+    for (int i = 0; i < op.nP; i++) {
+      op.pa[i] = (int8_t)pos_in(LA, LC[i]);
+      op.pb[i] = (int8_t)pos_in(LB, LC[i]);
+      if (op.pa[i] < 0 && op.pb[i] < 0) return fail(err, "output label missing from inputs");
+    }
+    // reduced labels, ordered by their position in A then B
+    std::vector<int32_t> R;
+    for (int32_t l : LA)
+      if (pos_in(LC, l) < 0) R.push_back(l);
+    for (int32_t l : LB)
+      if (pos_in(LC, l) < 0 && pos_in(LA, l) < 0) R.push_back(l);
+    if (R.size() > 8 && op.nP < 20) {
+      // split-K: keep the last s reduced labels in a temporary (more lanes,
+      // shorter loops), then reduce them in a second step
+      const int s_keep = std::min((int)R.size() - 4, 20 - op.nP);
+      std::vector<int32_t> tl = LC;
+      tl.insert(tl.end(), R.end() - s_keep, R.end());
+      const int t = add_tensor(tl);
+      if (ew(a, b, t)) return -1;
+      return ew(t, -1, c);
+    }
+    op.nR = (int)R.size();
+    if (op.nR > 16) return fail(err, "more than 16 reduced labels in one EW step");
+    for (int i = 0; i < op.nR; i++) {
+      op.ra[i] = (int8_t)pos_in(LA, R[i]);
+      op.rb[i] = (int8_t)pos_in(LB, R[i]);
+    }
+    op.ept = std::max(0, std::min(4, op.nP - 6 - std::max(0, op.nR - 2)));
+    if (op.nP - 6 < op.ept) op.ept = std::max(0, op.nP - 6);
+    op.tiles = op.nP >= 6 ? (int64_t)1 << (op.nP - 6 - op.ept) : 1;
+    op.macs = std::ldexp(1.0, op.nP + op.nR);
+    op.bytes = 4.0 * (std::ldexp(1.0, (int)LA.size()) + (b >= 0 ? std::ldexp(1.0, (int)LB.size()) : 0) +
+                      std::ldexp(1.0, op.nP));
+    P.ops.push_back(op);
+    P.n_ew++;
+    P.ew_bytes += op.bytes;
+    return 0;
+  }
+
+  // GEMM step: A in (a,k,t), B in (k,b,t), C in (a,b,t)
+  int gemm(int a, int b, int c, int na, int nb, int nk, int nt) {
+    POp op;
+    op.kind = kGemm;
+    op.a = a;
+    op.b = b;
+    op.c = c;
+    op.M = 1 << na;
+    op.N = 1 << nb;
+    op.K = 1 << nk;
+    op.batch = 1 << nt;
+    op.macs = std::ldexp(1.0, na + nb + nk + nt);
+    op.bytes = 4.0 * (std::ldexp(1.0, na + nk + nt) + std::ldexp(1.0, nk + nb + nt) +
+                      std::ldexp(1.0, na + nb + nt));
+    P.ops.push_back(op);
+    P.n_gemm++;
+    P.gemm_macs += op.macs;
+    return 0;
+  }
+};
+
+// Interval reuse of arena blocks: blocks freed by tensors that died at a level
+// strictly below a tensor's birth may be reused by it (exact size classes).
+int64_t assign_offsets(std::vector<PTensor>& T) {
+  std::vector<int> idx(T.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return T[x].birth < T[y].birth; });
+  std::vector<int> by_death(idx);
+  std::stable_sort(by_death.begin(), by_death.end(),
+                   [&](int x, int y) { return T[x].death < T[y].death; });
+  std::map<int64_t, std::vector<int64_t>> freel;  // class bytes -> offsets
+  int64_t top = 0;
+  size_t di = 0;
+  for (int t : idx) {
+    while (di < by_death.size() && T[by_death[di]].death < T[t].birth) {
+      const PTensor& d = T[by_death[di]];
+      if (d.off >= 0) freel[std::max<int64_t>(d.bytes, 16)].push_back(d.off);
+      di++;
+    }
+    int64_t cls = std::max<int64_t>(T[t].bytes, 16);
+    auto it = freel.find(cls);
+    if (it != freel.end() && !it->second.empty()) {
+      T[t].off = it->second.back();
+      it->second.pop_back();
+    } else {
+      int64_t al = std::min<int64_t>(cls, 256);
+      top = (top + al - 1) / al * al;
+      T[t].off = top;
+      top += cls;
+    }
+  }
+  return (top + 255) / 256 * 256;
+}
+
+// Every address a step can form stays inside its tensors and the arena:
+// positions are distinct and below the operand's rank, GEMM extents match the
+// tensor sizes, tiles cover exactly the output, and tensors fit the arena.
+int verify_plan(const SlicePlan& P, std::string* err) {
+  auto bad = [&](const std::string& m) { return fail(err, "plan check: " + m); };
+  for (const auto& t : P.tensors) {
+    if (t.off < 0 || t.off % 16 || t.off + std::max<int64_t>(t.bytes, 16) > P.arena_bytes)
+      return bad("tensor outside the arena");
+    if (t.rank < 0 || t.rank > kMaxRank) return bad("rank");
+  }
+  for (const auto& op : P.ops) {
+    const int ra = P.tensors[op.a].rank;
+    const int rb = op.b >= 0 ? P.tensors[op.b].rank : 0;
+    const int rc = P.tensors[op.c].rank;
+    if (op.kind == kEw) {
+      if (op.nP != rc || op.nR < 0 || op.nR > 16) return bad("EW ranks");
+      uint64_t ua = 0, ub = 0;
+      auto take = [&](int8_t p, int r, uint64_t& used) {
+        if (p < 0) return true;
+        if (p >= r || ((used >> p) & 1)) return false;
+        used |= 1ull << p;
+        return true;
+      };
+      for (int i = 0; i < op.nP; i++)
+        if (!take(op.pa[i], ra, ua) || !take(op.pb[i], rb, ub) || (op.b < 0 && op.pb[i] >= 0))
+          return bad("EW output positions");
+      for (int i = 0; i < op.nR; i++)
+        if (!take(op.ra[i], ra, ua) || !take(op.rb[i], rb, ub) || (op.b < 0 && op.rb[i] >= 0))
+          return bad("EW reduced positions");
+      if (ua != (ra >= 64 ? ~0ull : ((1ull << ra) - 1)) ||
+          (op.b >= 0 && ub != ((1ull << rb) - 1)))
+        return bad("EW step leaves an operand label unused");
+      const int tb = 6 + op.ept;
+      if (op.ept < 0 || op.ept > 4 || (op.ept > 0 && op.nP < tb)) return bad("EW ept");
+      if (op.tiles != (op.nP >= 6 ? (int64_t)1 << (op.nP - tb) : 1)) return bad("EW tiles");
+    } else {
+      const int64_t ea = (int64_t)op.M * op.K * op.batch, eb = (int64_t)op.K * op.N * op.batch,
+                    ec = (int64_t)op.M * op.N * op.batch;
+      if (op.b < 0 || ea != ((int64_t)1 << ra) || eb != ((int64_t)1 << rb) ||
+          ec != ((int64_t)1 << rc))
+        return bad("GEMM extents");
+      if (op.M % 4 || op.K % 4) return bad("GEMM vector alignment");
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+int slice_labels(const SliceIn& s, std::vector<std::vector<int32_t>>* labels, std::string* err) {
+  if (validate(s, err)) return -1;
+  compute_labels(s, *labels);
+  return 0;
+}
+
+int slice_complexity(const SliceIn& s, double* tc, double* sc, double* rwc, std::string* err) {
+  if (validate(s, err)) return -1;
+  if (s.nleaves < 2) {
+    *tc = -INFINITY;
+    *sc = s.nleaves == 1 ? s.ix_ptr[1] - s.ix_ptr[0] : 0;
+    *rwc = -INFINITY;
+    return 0;
+  }
+  std::vector<std::vector<int32_t>> lab;
+  compute_labels(s, lab);
+  double t = 0, rw = 0;
+  size_t scm = 0;
+  std::vector<char> mark(std::max(1, s.nv), 0);
+  for (int32_t j = 0; j + 1 < s.nleaves; j++) {
+    const auto &A = lab[s.tree[2 * j]], &B = lab[s.tree[2 * j + 1]], &C = lab[s.nleaves + j];
+    int u = 0;
+    for (int32_t l : A) mark[l] = 1, u++;
+    for (int32_t l : B)
+      if (!mark[l]) u++;
+    for (int32_t l : A) mark[l] = 0;
+    t += std::ldexp(1.0, u);
+    rw += std::ldexp(1.0, (int)A.size()) + std::ldexp(1.0, (int)B.size()) + std::ldexp(1.0, (int)C.size());
+    scm = std::max(scm, C.size());
+  }
+  *tc = std::log2(t);
+  *sc = (double)scm;
+  *rwc = std::log2(rw);
+  return 0;
+}
+
+// contraction_peak_memory and contraction_all_memory, src/utils.jl:197-229.
+int slice_peak_memory(const SliceIn& s, double* peak, double* all, std::string* err) {
+  if (validate(s, err)) return -1;
+  if (s.nleaves == 0) {
+    *peak = -INFINITY;
+    *all = -INFINITY;
+    return 0;
+  }
+  std::vector<std::vector<int32_t>> lab;
+  compute_labels(s, lab);
+  int32_t nl = s.nleaves, nn = nl - 1;
+  double cur = 0;
+  for (int32_t i = 0; i < nl; i++) cur += std::ldexp(1.0, (int)lab[i].size());
+  double pk = cur, tot = 0;
+  std::vector<double> ffs(std::max(1, nn), 0.0);
+  for (int32_t j = 0; j < nn; j++) {
+    double freed = 0;
+    for (int q = 0; q < 2; q++) {
+      int32_t c = s.tree[2 * j + q];
+      if (c >= nl) freed += ffs[c - nl];
+      ffs[j] += std::ldexp(1.0, (int)lab[c].size());
+    }
+    double alloc = std::ldexp(1.0, (int)lab[nl + j].size());
+    tot += alloc;
+    cur += alloc - freed;
+    pk = std::max(pk, cur);
+  }
+  *peak = std::log2(pk);
+  *all = nn > 0 ? std::log2(tot) : -INFINITY;
+  return 0;
+}
+
+int plan_slice(const SliceIn& s, const PlanOptions& opt, SlicePlan* out, std::string* err) {
+  SlicePlan& P = *out;
+  P = SlicePlan();
+  if (validate(s, err)) return -1;
+  if (s.nv == 0 || s.nleaves == 0) {
+    P.empty = true;
+    return 0;
+  }
+  P.empty = false;
+  const int32_t nl = s.nleaves, nn = nl - 1;
+  std::vector<std::vector<int32_t>> lab;
+  compute_labels(s, lab);
+
+  // elimination step of each label (node index where it disappears)
+  std::vector<int32_t> elim(s.nv, nn);
+  for (int32_t j = 0; j < nn; j++) {
+    const auto &A = lab[s.tree[2 * j]], &B = lab[s.tree[2 * j + 1]], &C = lab[nl + j];
+    for (const auto* X : {&A, &B})
+      for (int32_t l : *X)
+        if (pos_in(C, l) < 0) elim[l] = j;
+  }
+  auto sorted_key = [&](std::vector<int32_t> v) {
+    std::sort(v.begin(), v.end(), [&](int32_t x, int32_t y) {
+      return elim[x] != elim[y] ? elim[x] > elim[y] : x < y;
+    });
+    return v;
+  };
+
+  Builder B{s, opt, P, {}, err};
+  // leaves are tensors 0..nl-1 (content is symmetric, so any layout works)
+  for (int32_t i = 0; i < nl; i++) {
+    B.add_tensor(opt.batched ? sorted_key(lab[i]) : lab[i]);
+    PLeaf L;
+    L.tensor = i;
+    L.kind = (int)lab[i].size();
+    L.wbits = 0;
+    if (L.kind == 1) {
+      int32_t v = lab[i][0];
+      L.wbits = s.wbits ? s.wbits[v] : s.unit_wbits;
+    }
+    P.leaves.push_back(L);
+  }
+  // internal node output tensors nl..nl+nn-1 (layouts fixed below)
+  for (int32_t j = 0; j < nn; j++) B.add_tensor(lab[nl + j]);
+
+  // classify labels of node j
+  struct Cls {
+    std::vector<int32_t> a, b, k, t;  // a: A-only out, b: B-only out
+    bool priv = false;
+    int ca = -1, cb = -1;             // child tensors in A / B role
+  };
+  auto classify = [&](int32_t j) {
+    Cls c;
+    int32_t x = s.tree[2 * j], y = s.tree[2 * j + 1];
+    const auto &LX = lab[x], &LY = lab[y], &LC = lab[nl + j];
+    std::vector<int32_t> fx, fy;
+    for (int32_t l : LC) {
+      bool inx = pos_in(LX, l) >= 0, iny = pos_in(LY, l) >= 0;
+      if (inx && iny)
+        c.t.push_back(l);
+      else if (inx)
+        fx.push_back(l);
+      else
+        fy.push_back(l);
+    }
+    for (int32_t l : LX) {
+      if (pos_in(LC, l) >= 0) continue;
+      if (pos_in(LY, l) >= 0)
+        c.k.push_back(l);
+      else
+        c.priv = true;
+    }
+    for (int32_t l : LY)
+      if (pos_in(LC, l) < 0 && pos_in(LX, l) < 0) c.priv = true;
+    // A role: the side with more free labels (M >= N)
+    if (fx.size() >= fy.size()) {
+      c.ca = x;
+      c.cb = y;
+      c.a = fx;
+      c.b = fy;
+    } else {
+      c.ca = y;
+      c.cb = x;
+      c.a = fy;
+      c.b = fx;
+    }
+    return c;
+  };
+  auto concat = [](std::initializer_list<const std::vector<int32_t>*> parts) {
+    std::vector<int32_t> r;
+    for (auto* p : parts) r.insert(r.end(), p->begin(), p->end());
+    return r;
+  };
+
+  // tree-node contraction count (the reference's tc)
+  for (int32_t j = 0; j < nn; j++) {
+    const auto &A = lab[s.tree[2 * j]], &Bv = lab[s.tree[2 * j + 1]];
+    int u = (int)A.size();
+    for (int32_t l : Bv)
+      if (pos_in(A, l) < 0) u++;
+    P.node_macs += std::ldexp(1.0, u);
+  }
+
+  if (opt.batched) {
+    // top-down layout requests; GEMM-shaped nodes ask their children for
+    // (a,k,t) / (k,b,t) and write (a,b,t)
+    std::vector<char> is_gemm(nn, 0), has_req(nl + nn, 0);
+    std::vector<std::vector<int32_t>> req(nl + nn);
+    std::vector<Cls> cls(nn);
+    std::vector<std::vector<int32_t>> gemm_out(nn);
+    for (int32_t j = nn - 1; j >= 0; j--) {
+      cls[j] = classify(j);
+      Cls& c = cls[j];
+      const int o = nl + j;
+      int na = (int)c.a.size(), nb = (int)c.b.size(), nk = (int)c.k.size();
+      bool g = !c.priv && (1 << std::min(na, 30)) >= opt.gemm_min_m &&
+               (1 << std::min(nb, 30)) >= opt.gemm_min_n && (1 << std::min(nk, 30)) >= opt.gemm_min_k;
+      is_gemm[j] = g;
+      if (g) {
+        auto sa = sorted_key(c.a), sb = sorted_key(c.b), sk = sorted_key(c.k), st = sorted_key(c.t);
+        c.a = sa;
+        c.b = sb;
+        c.k = sk;
+        c.t = st;
+        gemm_out[j] = concat({&sa, &sb, &st});
+        req[c.ca] = concat({&sa, &sk, &st});
+        req[c.cb] = concat({&sk, &sb, &st});
+        has_req[c.ca] = has_req[c.cb] = 1;
+        B.layout[o] = has_req[o] ? req[o] : gemm_out[j];
+      } else {
+        B.layout[o] = has_req[o] ? req[o] : sorted_key(lab[o]);
+      }
+    }
+    for (int32_t i = 0; i < nl; i++)
+      if (has_req[i]) B.layout[i] = req[i];
+    // emit steps in post-order
+    for (int32_t j = 0; j < nn; j++) {
+      const int o = nl + j;
+      Cls& c = cls[j];
+      if (is_gemm[j]) {
+        int dst = o;
+        if (B.layout[o] != gemm_out[j]) dst = B.add_tensor(gemm_out[j]);
+        B.gemm(c.ca, c.cb, dst, (int)c.a.size(), (int)c.b.size(), (int)c.k.size(), (int)c.t.size());
+        if (dst != o && B.ew(dst, -1, o)) return -1;
+      } else {
+        if (B.ew(s.tree[2 * j], s.tree[2 * j + 1], o)) return -1;
+      }
+    }
+  } else {
+    // OMEinsum lowering, node outputs in parse_eincode label order
+    for (int32_t j = 0; j < nn; j++) {
+      const int o = nl + j;
+      Cls c = classify(j);
+      int x = s.tree[2 * j], y = s.tree[2 * j + 1];
+      // groups as OMEinsum forms them: a, b, t follow the output, k input A
+      c.ca = x;
+      c.cb = y;
+      c.a.clear();
+      c.b.clear();
+      c.k.clear();
+      c.t.clear();
+      const auto &LX = lab[x], &LY = lab[y], &LC = lab[o];
+      for (int32_t l : LC) {
+        bool inx = pos_in(LX, l) >= 0, iny = pos_in(LY, l) >= 0;
+        if (inx && iny)
+          c.t.push_back(l);
+        else if (inx)
+          c.a.push_back(l);
+        else
+          c.b.push_back(l);
+      }
+      for (int32_t l : LX)
+        if (pos_in(LC, l) < 0 && pos_in(LY, l) >= 0) c.k.push_back(l);
+      auto wantA = concat({&c.a, &c.k, &c.t}), wantB = concat({&c.k, &c.b, &c.t});
+      auto gout = concat({&c.a, &c.b, &c.t});
+      int ta = x, tb = y;
+      if (B.layout[x] != wantA) {
+        ta = B.add_tensor(wantA);
+        if (B.ew(x, -1, ta)) return -1;
+      }
+      if (B.layout[y] != wantB) {
+        tb = B.add_tensor(wantB);
+        if (B.ew(y, -1, tb)) return -1;
+      }
+      int dst = o;
+      if (gout != LC) dst = B.add_tensor(gout);
+      int na = (int)c.a.size(), nb = (int)c.b.size(), nk = (int)c.k.size();
+      bool g = (1 << std::min(na, 30)) >= opt.gemm_min_m && (1 << std::min(nb, 30)) >= opt.gemm_min_n &&
+               (1 << std::min(nk, 30)) >= opt.gemm_min_k;
+      if (g)
+        B.gemm(ta, tb, dst, na, nb, nk, (int)c.t.size());
+      else if (B.ew(ta, tb, dst))
+        return -1;
+      if (dst != o && B.ew(dst, -1, o)) return -1;
+    }
+  }
+
+  // root: the last node's output is a scalar; a single leaf is reduced
+  if (nn == 0) {
+    int sc = B.add_tensor({});
+    if (B.ew(0, -1, sc)) return -1;
+    P.root = sc;
+  } else {
+    P.root = nl + nn - 1;
+  }
+  if (P.tensors[P.root].rank != 0) return fail(err, "root tensor is not a scalar");
+
+  // producers and levels
+  const int nt = (int)P.tensors.size();
+  std::vector<int> producer(nt, -1), consumer(nt, -1);
+  for (size_t i = 0; i < P.ops.size(); i++) {
+    producer[P.ops[i].c] = (int)i;
+    for (int in : {P.ops[i].a, P.ops[i].b})
+      if (in >= 0) consumer[in] = (int)i;
+  }
+  const int nops = (int)P.ops.size();
+  if (opt.batched) {
+    std::vector<int> asap(nops, 1);
+    int H = 0;
+    for (int i = 0; i < nops; i++) {
+      int lv = 1;
+      for (int in : {P.ops[i].a, P.ops[i].b})
+        if (in >= 0 && producer[in] >= 0) lv = std::max(lv, asap[producer[in]] + 1);
+      asap[i] = lv;
+      H = std::max(H, lv);
+    }
+    // ALAP: as late as the consumer allows (side subtrees just in time)
+    for (int i = nops - 1; i >= 0; i--) {
+      int cons = consumer[P.ops[i].c];
+      P.ops[i].level = cons < 0 ? H : P.ops[cons].level - 1;
+    }
+    P.nlevels = H;
+    for (int t = 0; t < nt; t++) {
+      P.tensors[t].birth = producer[t] >= 0 ? P.ops[producer[t]].level : 0;
+      P.tensors[t].death = consumer[t] >= 0 ? P.ops[consumer[t]].level : H + 1;
+    }
+  } else {
+    for (int i = 0; i < nops; i++) P.ops[i].level = i + 1;
+    P.nlevels = nops;
+    for (int t = 0; t < nt; t++) {
+      P.tensors[t].birth = producer[t] >= 0 ? producer[t] + 1 : 0;
+      P.tensors[t].death = consumer[t] >= 0 ? consumer[t] + 1 : nops + 1;
+    }
+  }
+  for (auto& t : P.tensors) t.off = -1;
+  P.arena_bytes = assign_offsets(P.tensors);
+  if (verify_plan(P, err)) return -1;
+  for (const auto& op : P.ops) {
+    P.exec_macs += op.macs;
+    P.exec_bytes += op.bytes;
+  }
+  return 0;
+}
+
+}  // namespace tbh

@@ -1,0 +1,81 @@
+// tbh_plan.hpp -- host planner: lowers one slice's contraction tree to the
+// steps the gfx950 kernels run, with tensor layouts and an arena layout.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "tbh_internal.hpp"
+
+namespace tbh {
+
+// One slice as handed over through the C ABI (tbh_slice), weights already
+// converted to the dtype's 32-bit pattern (nullptr = unit weights).
+struct SliceIn {
+  int32_t nv = 0;
+  const uint32_t* wbits = nullptr;
+  uint32_t unit_wbits = 0;   // bit pattern of weight 1 in the dtype
+  int32_t nleaves = 0;
+  const int32_t* ix_ptr = nullptr;
+  const int32_t* ix = nullptr;
+  const int32_t* tree = nullptr;
+  double r = 0;
+};
+
+struct PTensor {
+  int rank = 0;
+  int64_t bytes = 0;
+  int64_t off = 0;   // arena byte offset
+  int birth = 0, death = 0;
+};
+
+enum OpKind { kEw = 0, kGemm = 1 };
+
+struct POp {
+  int kind = kEw;
+  int a = -1, b = -1, c = -1;  // tensor ids (b = -1: unary)
+  int level = 0;               // wave (batched) or sequence index (plain)
+  // EW
+  int nP = 0, nR = 0, ept = 0;
+  int64_t tiles = 0;
+  int8_t pa[32], pb[32], ra[16], rb[16];
+  // GEMM
+  int32_t M = 0, N = 0, K = 0, batch = 0;
+  double macs = 0;     // executed (output x reduced) configurations
+  double bytes = 0;    // algorithmic bytes: |A| + |B| + |C| elements x 4
+};
+
+struct PLeaf {
+  int tensor;
+  int kind;        // 1 vertex, 2 edge
+  uint32_t wbits;
+};
+
+struct SlicePlan {
+  bool empty = true;     // no leaves: the contraction is the tropical one
+  std::vector<PTensor> tensors;
+  std::vector<POp> ops;  // in dependency order
+  std::vector<PLeaf> leaves;
+  int root = -1;         // tensor holding the scalar result
+  int nlevels = 0;       // ops use levels 1..nlevels
+  int64_t arena_bytes = 0;
+  double node_macs = 0;  // sum over tree nodes of 2^|union| (= 2^tc)
+  double exec_macs = 0, exec_bytes = 0, gemm_macs = 0, ew_bytes = 0;
+  int n_ew = 0, n_gemm = 0;
+};
+
+struct PlanOptions {
+  bool batched = true;   // wave levels + layout planning; false: OMEinsum lowering
+  int gemm_min_m = 64, gemm_min_n = 64, gemm_min_k = 16;
+};
+
+// Returns 0 or sets *err and returns -1.
+int plan_slice(const SliceIn& s, const PlanOptions& opt, SlicePlan* out, std::string* err);
+
+// Reference-model helpers (every label of size 2).
+int slice_labels(const SliceIn& s, std::vector<std::vector<int32_t>>* labels, std::string* err);
+int slice_complexity(const SliceIn& s, double* tc, double* sc, double* rwc, std::string* err);
+int slice_peak_memory(const SliceIn& s, double* peak, double* all, std::string* err);
+
+}  // namespace tbh

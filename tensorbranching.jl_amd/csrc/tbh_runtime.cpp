@@ -1,0 +1,892 @@
+// tbh_runtime.cpp -- context, device pool, executors and the C ABI of
+// libtbh.so (include/tbh.h).
+//
+// contract_slices (src/dynamic_ob.jl:36-48) becomes: convert and validate the
+// slices, plan each one on host threads, then either
+//  - PLAIN: per slice, upload its steps, launch them one by one, read the
+//    scalar back (the per-slice GPU loop of usecuda=true), or
+//  - BATCHED: pack as many slices as the pool holds into one arena, upload
+//    every step of the batch once, and run one launch per wave and kernel
+//    kind across all slices, then one epilogue launch and one copy back.
+#include "tbh_runtime.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace tbh {
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+
+#define HIP_TRY(x)                                                          \
+  do {                                                                      \
+    hipError_t e_ = (x);                                                    \
+    if (e_ != hipSuccess) {                                                 \
+      set_error(std::string(#x) + ": " + hipGetErrorString(e_));            \
+      return TBH_EDEVICE;                                                   \
+    }                                                                       \
+  } while (0)
+
+// ------------------------------------------------------------ pool -------
+void Pool::init(size_t bytes) {
+  cap_ = bytes / 256 * 256;
+  free_.clear();
+  inuse_.clear();
+  free_[0] = cap_;
+  used_ = peak_ = 0;
+}
+
+int64_t Pool::alloc(size_t bytes) {
+  bytes = std::max<size_t>(256, (bytes + 255) / 256 * 256);
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second >= bytes) {
+      int64_t off = it->first;
+      size_t rest = it->second - bytes;
+      free_.erase(it);
+      if (rest) free_[off + (int64_t)bytes] = rest;
+      inuse_[off] = bytes;
+      used_ += bytes;
+      peak_ = std::max(peak_, used_);
+      return off;
+    }
+  }
+  return -1;
+}
+
+void Pool::free(int64_t off) {
+  auto it = inuse_.find(off);
+  if (it == inuse_.end()) return;
+  size_t sz = it->second;
+  inuse_.erase(it);
+  used_ -= sz;
+  auto nx = free_.lower_bound(off);
+  if (nx != free_.end() && nx->first == off + (int64_t)sz) {
+    sz += nx->second;
+    nx = free_.erase(nx);
+  }
+  if (nx != free_.begin()) {
+    auto pv = std::prev(nx);
+    if (pv->first + (int64_t)pv->second == off) {
+      pv->second += sz;
+      return;
+    }
+  }
+  free_[off] = sz;
+}
+
+size_t Pool::largest_free() const {
+  size_t m = 0;
+  for (auto& kv : free_) m = std::max(m, kv.second);
+  return m;
+}
+
+int Staging::ensure(size_t bytes) {
+  if (bytes <= cap) return 0;
+  if (host) (void)hipHostFree(host);
+  host = nullptr;
+  cap = 0;
+  size_t want = std::max<size_t>(bytes, (size_t)1 << 20);
+  if (hipHostMalloc(&host, want, hipHostMallocDefault) != hipSuccess) {
+    host = nullptr;
+    set_error("hipHostMalloc of staging buffer failed");
+    return TBH_ENOMEM;
+  }
+  cap = want;
+  return 0;
+}
+
+Staging::~Staging() {
+  if (host) (void)hipHostFree(host);
+}
+
+// ------------------------------------------------------- conversion ------
+static uint32_t f2b(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+
+int convert_slice(const tbh_slice& s, tbh_dtype dtype, bool copy, SliceOwned* out) {
+  if (dtype != TBH_F32 && dtype != TBH_I32) {
+    set_error("unknown dtype");
+    return TBH_EINVAL;
+  }
+  if (s.nv < 0 || s.nleaves < 0) {
+    set_error("negative nv or nleaves");
+    return TBH_EINVAL;
+  }
+  SliceOwned& o = *out;
+  o.empty_graph = s.nv == 0;
+  if (dtype == TBH_F32) {
+    o.rbits = f2b((float)s.r);
+  } else {
+    if (s.r != std::floor(s.r) || std::fabs(s.r) >= (double)(1 << 28)) {
+      set_error("Int32 slices need an integer r with |r| < 2^28");
+      return TBH_EINVAL;
+    }
+    o.rbits = (uint32_t)(int32_t)s.r;
+  }
+  o.in.nv = s.nv;
+  o.in.nleaves = s.nv == 0 ? 0 : s.nleaves;
+  o.in.r = s.r;
+  o.in.unit_wbits = dtype == TBH_F32 ? f2b(1.0f) : 1u;
+  o.wbits.clear();
+  if (s.weights && s.nv > 0) {
+    o.wbits.resize(s.nv);
+    std::memcpy(o.wbits.data(), s.weights, 4 * (size_t)s.nv);
+    if (dtype == TBH_I32)
+      for (int32_t v = 0; v < s.nv; v++) {
+        int32_t w = (int32_t)o.wbits[v];
+        if (w <= -(1 << 28) || w >= (1 << 28)) {
+          set_error("Int32 weights must satisfy |w| < 2^28");
+          return TBH_EINVAL;
+        }
+      }
+    o.in.wbits = o.wbits.data();
+  } else {
+    o.in.wbits = nullptr;
+  }
+  if (o.in.nleaves > 0) {
+    if (!s.ix_ptr || !s.ix || (o.in.nleaves > 1 && !s.tree)) {
+      set_error("slice without ix_ptr / ix / tree");
+      return TBH_EINVAL;
+    }
+    if (copy) {
+      o.ix_ptr.assign(s.ix_ptr, s.ix_ptr + o.in.nleaves + 1);
+      o.ix.assign(s.ix, s.ix + o.ix_ptr.back());
+      if (o.in.nleaves > 1) o.tree.assign(s.tree, s.tree + 2 * (size_t)(o.in.nleaves - 1));
+      o.in.ix_ptr = o.ix_ptr.data();
+      o.in.ix = o.ix.data();
+      o.in.tree = o.tree.data();
+    } else {
+      o.in.ix_ptr = s.ix_ptr;
+      o.in.ix = s.ix;
+      o.in.tree = s.tree;
+    }
+  }
+  return 0;
+}
+
+int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool batched,
+             std::vector<SlicePlan>* plans) {
+  plans->assign(slices.size(), SlicePlan());
+  PlanOptions opt;
+  opt.batched = batched;
+  std::atomic<size_t> next{0};
+  std::atomic<int> bad{-1};
+  std::string first_err;
+  std::mutex em;
+  auto worker = [&]() {
+    for (;;) {
+      size_t i = next.fetch_add(1);
+      if (i >= slices.size() || bad.load() >= 0) return;
+      std::string err;
+      if (plan_slice(slices[i]->in, opt, &(*plans)[i], &err)) {
+        std::lock_guard<std::mutex> g(em);
+        if (bad.load() < 0) {
+          bad = (int)i;
+          first_err = "slice " + std::to_string(i) + ": " + err;
+        }
+      }
+    }
+  };
+  int nt = (int)std::min<size_t>(std::max(1, ctx->plan_threads), slices.size());
+  if (nt <= 1) {
+    worker();
+  } else {
+    std::vector<std::thread> th;
+    for (int k = 0; k < nt; k++) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+  }
+  if (bad.load() >= 0) {
+    set_error(first_err);
+    return TBH_EINVAL;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------- descriptor packs ---
+namespace {
+
+struct Pack {
+  std::vector<char> buf;
+  size_t put(const void* p, size_t n) {
+    size_t off = (buf.size() + 255) / 256 * 256;
+    buf.resize(off + n);
+    if (n) std::memcpy(buf.data() + off, p, n);
+    return off;
+  }
+  size_t reserve(size_t n) {
+    size_t off = (buf.size() + 255) / 256 * 256;
+    buf.resize(off + n);
+    return off;
+  }
+};
+
+EwDesc make_ew(const POp& op, const std::vector<PTensor>& T, uint64_t base) {
+  EwDesc d;
+  std::memset(&d, 0, sizeof d);
+  d.a = base + (uint64_t)T[op.a].off;
+  d.b = op.b >= 0 ? base + (uint64_t)T[op.b].off : 0;
+  d.c = base + (uint64_t)T[op.c].off;
+  d.nP = op.nP;
+  d.nR = op.nR;
+  d.ept = op.ept;
+  d.tiles = (int32_t)op.tiles;
+  std::memcpy(d.pa, op.pa, 32);
+  std::memcpy(d.pb, op.pb, 32);
+  std::memcpy(d.ra, op.ra, 16);
+  std::memcpy(d.rb, op.rb, 16);
+  return d;
+}
+
+GemmDesc make_gemm(const POp& op, const std::vector<PTensor>& T, uint64_t base) {
+  GemmDesc d;
+  std::memset(&d, 0, sizeof d);
+  d.a = base + (uint64_t)T[op.a].off;
+  d.b = base + (uint64_t)T[op.b].off;
+  d.c = base + (uint64_t)T[op.c].off;
+  d.M = op.M;
+  d.N = op.N;
+  d.K = op.K;
+  d.batch = op.batch;
+  d.lda = op.M;
+  d.ldb = op.K;
+  d.ldc = op.M;
+  d.sA = (int64_t)op.M * op.K;
+  d.sB = (int64_t)op.K * op.N;
+  d.sC = (int64_t)op.M * op.N;
+  d.tiles_m = (op.M + 127) / 128;
+  d.tiles_n = (op.N + 127) / 128;
+  return d;
+}
+
+int64_t gemm_tiles(const GemmDesc& d) { return (int64_t)d.tiles_m * d.tiles_n * d.batch; }
+
+int ew_grid(tbh_ctx* ctx, int64_t total, int chunk) {
+  int64_t waves = (total + chunk - 1) / chunk;
+  int64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)ctx->num_cus * 8));
+}
+
+int gemm_grid(tbh_ctx* ctx, int64_t total) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)ctx->num_cus * 2));
+}
+
+constexpr int kChunk = 4;
+
+}  // namespace
+
+// ------------------------------------------------------ BATCHED ----------
+int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*>& slices,
+                    const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* results) {
+  const size_t n = slices.size();
+  size_t i0 = 0;
+  while (i0 < n) {
+    // admit slices while their arenas fit (leave room for descriptors)
+    size_t budget = ctx->pool.largest_free();
+    budget = budget > ((size_t)64 << 20) ? budget - ((size_t)64 << 20) : 0;
+    size_t i1 = i0, arena = 0, ndesc_bytes = 0;
+    while (i1 < n) {
+      const SlicePlan& P = *plans[i1];
+      size_t a = P.empty ? 0 : (size_t)P.arena_bytes;
+      size_t d = P.ops.size() * (sizeof(EwDesc) + 16) + P.leaves.size() * sizeof(LeafDesc) + 512;
+      if (i1 > i0 && arena + a + 2 * (ndesc_bytes + d) > budget) break;
+      arena += a;
+      ndesc_bytes += d;
+      i1++;
+    }
+    if (arena + 2 * ndesc_bytes > budget) {
+      set_error("slice arena (" + std::to_string(arena) + " bytes) does not fit the pool");
+      return TBH_ENOMEM;
+    }
+    int64_t arena_off = ctx->pool.alloc(std::max<size_t>(arena, 256));
+    if (arena_off < 0) {
+      set_error("pool exhausted");
+      return TBH_ENOMEM;
+    }
+    const uint64_t arena_base = (uint64_t)(ctx->base + arena_off);
+    // per-slice bases
+    std::vector<uint64_t> sbase(i1 - i0);
+    {
+      uint64_t acc = 0;
+      for (size_t i = i0; i < i1; i++) {
+        sbase[i - i0] = arena_base + acc;
+        acc += plans[i]->empty ? 0 : (uint64_t)plans[i]->arena_bytes;
+      }
+    }
+    int H = 0;
+    for (size_t i = i0; i < i1; i++) H = std::max(H, plans[i]->nlevels);
+    // gather steps by level
+    std::vector<std::vector<EwDesc>> ew(H + 1);
+    std::vector<std::vector<GemmDesc>> gm(H + 1);
+    std::vector<LeafDesc> leaves;
+    std::vector<EpiDesc> epi(i1 - i0);
+    for (size_t i = i0; i < i1; i++) {
+      const SlicePlan& P = *plans[i];
+      const uint64_t b = sbase[i - i0];
+      EpiDesc& e = epi[i - i0];
+      std::memset(&e, 0, sizeof e);
+      e.rbits = add_r ? slices[i]->rbits : (dtype == TBH_F32 ? f2b(0.0f) : 0u);
+      if (P.empty) {
+        e.root = 0;
+        if (!add_r) e.rbits = dtype == TBH_F32 ? f2b(0.0f) : 0u;
+        continue;
+      }
+      e.root = b + (uint64_t)P.tensors[P.root].off;
+      for (const PLeaf& L : P.leaves) {
+        LeafDesc d;
+        d.p = b + (uint64_t)P.tensors[L.tensor].off;
+        d.kind = L.kind;
+        d.wbits = (int32_t)L.wbits;
+        leaves.push_back(d);
+      }
+      for (const POp& op : P.ops) {
+        if (op.kind == kEw) {
+          ew[op.level].push_back(make_ew(op, P.tensors, b));
+        } else {
+          gm[op.level].push_back(make_gemm(op, P.tensors, b));
+        }
+        ctx->stats.ew_ops += op.kind == kEw;
+        ctx->stats.gemm_ops += op.kind == kGemm;
+        if (op.kind == kEw) ctx->stats.ew_ops_algo_bytes += op.bytes;
+        if (op.kind == kGemm) ctx->stats.gemm_ops_algo_ops += 2 * op.macs;
+      }
+      ctx->stats.total_algo_ops += 2 * P.node_macs;
+      ctx->stats.leaf_tensors += P.leaves.size();
+    }
+    // pack
+    Pack pk;
+    const size_t o_leaves = pk.put(leaves.data(), leaves.size() * sizeof(LeafDesc));
+    const size_t o_epi = pk.put(epi.data(), epi.size() * sizeof(EpiDesc));
+    const size_t o_res = pk.reserve(4 * (i1 - i0));
+    struct LvlOff {
+      size_t ew, ewp, gm, gmp;
+      int64_t ew_total, gm_total;
+    };
+    std::vector<LvlOff> lo(H + 1);
+    for (int L = 1; L <= H; L++) {
+      std::vector<int64_t> pe(ew[L].size() + 1, 0), pg(gm[L].size() + 1, 0);
+      for (size_t k = 0; k < ew[L].size(); k++) pe[k + 1] = pe[k] + ew[L][k].tiles;
+      for (size_t k = 0; k < gm[L].size(); k++) pg[k + 1] = pg[k] + gemm_tiles(gm[L][k]);
+      lo[L].ew = pk.put(ew[L].data(), ew[L].size() * sizeof(EwDesc));
+      lo[L].ewp = pk.put(pe.data(), pe.size() * sizeof(int64_t));
+      lo[L].gm = pk.put(gm[L].data(), gm[L].size() * sizeof(GemmDesc));
+      lo[L].gmp = pk.put(pg.data(), pg.size() * sizeof(int64_t));
+      lo[L].ew_total = pe.back();
+      lo[L].gm_total = pg.back();
+    }
+    const size_t bytes = pk.buf.size();
+    int64_t doff = ctx->pool.alloc(bytes);
+    if (doff < 0) {
+      ctx->pool.free(arena_off);
+      set_error("pool exhausted (descriptors)");
+      return TBH_ENOMEM;
+    }
+    char* dbase = ctx->base + doff;
+    int rc = ctx->staging.ensure(bytes);
+    if (rc) {
+      ctx->pool.free(doff);
+      ctx->pool.free(arena_off);
+      return rc;
+    }
+    std::memcpy(ctx->staging.host, pk.buf.data(), bytes);
+    auto cleanup = [&]() {
+      ctx->pool.free(doff);
+      ctx->pool.free(arena_off);
+    };
+    auto check = [&](hipError_t e, const char* what) {
+      if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return true;
+      }
+      return false;
+    };
+    if (check(hipMemcpyAsync(dbase, ctx->staging.host, bytes, hipMemcpyHostToDevice, ctx->stream),
+              "upload")) {
+      (void)hipStreamSynchronize(ctx->stream);
+      cleanup();
+      return TBH_EDEVICE;
+    }
+    hipError_t e = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), (int64_t)leaves.size(),
+                                 ctx->stream);
+    ctx->stats.launches += leaves.empty() ? 0 : 1;
+    for (int L = 1; L <= H && e == hipSuccess; L++) {
+      if (lo[L].ew_total > 0) {
+        e = launch_ew(dtype, (const EwDesc*)(dbase + lo[L].ew), (const int64_t*)(dbase + lo[L].ewp),
+                      (int)ew[L].size(), lo[L].ew_total, ew_grid(ctx, lo[L].ew_total, kChunk),
+                      kChunk, ctx->stream);
+        ctx->stats.launches++;
+      }
+      if (e == hipSuccess && lo[L].gm_total > 0) {
+        e = launch_gemm(dtype, true, (const GemmDesc*)(dbase + lo[L].gm),
+                        (const int64_t*)(dbase + lo[L].gmp), (int)gm[L].size(), lo[L].gm_total,
+                        gemm_grid(ctx, lo[L].gm_total), ctx->stream);
+        ctx->stats.launches++;
+      }
+      ctx->stats.waves++;
+    }
+    if (e == hipSuccess) {
+      e = launch_epilogue(dtype, (const EpiDesc*)(dbase + o_epi), (int64_t)epi.size(),
+                          dbase + o_res, ctx->stream);
+      ctx->stats.launches++;
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(ctx->staging.host, dbase + o_res, 4 * (i1 - i0), hipMemcpyDeviceToHost,
+                         ctx->stream);
+    hipError_t es = hipStreamSynchronize(ctx->stream);
+    if (check(e, "batched launch") || check(es, "batched sync")) {
+      cleanup();
+      return TBH_EDEVICE;
+    }
+    std::memcpy(results + i0, ctx->staging.host, 4 * (i1 - i0));
+    cleanup();
+    i0 = i1;
+  }
+  return 0;
+}
+
+// -------------------------------------------------------- PLAIN ----------
+int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SlicePlan& P,
+                  bool add_r, uint32_t* result) {
+  const uint32_t zero_bits = dtype == TBH_F32 ? f2b(0.0f) : 0u;
+  if (P.empty) {
+    *result = add_r ? slice.rbits : zero_bits;
+    return 0;
+  }
+  int64_t arena_off = ctx->pool.alloc(std::max<int64_t>(P.arena_bytes, 256));
+  if (arena_off < 0) {
+    set_error("slice arena does not fit the pool");
+    return TBH_ENOMEM;
+  }
+  const uint64_t b = (uint64_t)(ctx->base + arena_off);
+  Pack pk;
+  std::vector<LeafDesc> leaves;
+  for (const PLeaf& L : P.leaves) {
+    LeafDesc d;
+    d.p = b + (uint64_t)P.tensors[L.tensor].off;
+    d.kind = L.kind;
+    d.wbits = (int32_t)L.wbits;
+    leaves.push_back(d);
+  }
+  EpiDesc e;
+  std::memset(&e, 0, sizeof e);
+  e.root = b + (uint64_t)P.tensors[P.root].off;
+  e.rbits = add_r ? slice.rbits : zero_bits;
+  const size_t o_leaves = pk.put(leaves.data(), leaves.size() * sizeof(LeafDesc));
+  const size_t o_epi = pk.put(&e, sizeof e);
+  const size_t o_res = pk.reserve(4);
+  std::vector<size_t> o_op(P.ops.size()), o_pre(P.ops.size());
+  std::vector<int64_t> tiles(P.ops.size());
+  for (size_t k = 0; k < P.ops.size(); k++) {
+    const POp& op = P.ops[k];
+    int64_t pre[2] = {0, 0};
+    if (op.kind == kEw) {
+      EwDesc d = make_ew(op, P.tensors, b);
+      o_op[k] = pk.put(&d, sizeof d);
+      pre[1] = d.tiles;
+    } else {
+      GemmDesc d = make_gemm(op, P.tensors, b);
+      o_op[k] = pk.put(&d, sizeof d);
+      pre[1] = gemm_tiles(d);
+    }
+    tiles[k] = pre[1];
+    o_pre[k] = pk.put(pre, sizeof pre);
+  }
+  const size_t bytes = pk.buf.size();
+  int64_t doff = ctx->pool.alloc(bytes);
+  if (doff < 0) {
+    ctx->pool.free(arena_off);
+    set_error("pool exhausted (descriptors)");
+    return TBH_ENOMEM;
+  }
+  char* dbase = ctx->base + doff;
+  int rc = ctx->staging.ensure(bytes);
+  if (rc) {
+    ctx->pool.free(doff);
+    ctx->pool.free(arena_off);
+    return rc;
+  }
+  std::memcpy(ctx->staging.host, pk.buf.data(), bytes);
+  hipError_t err =
+      hipMemcpyAsync(dbase, ctx->staging.host, bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (err == hipSuccess)
+    err = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), (int64_t)leaves.size(),
+                        ctx->stream);
+  ctx->stats.launches++;
+  for (size_t k = 0; k < P.ops.size() && err == hipSuccess; k++) {
+    const POp& op = P.ops[k];
+    if (op.kind == kEw) {
+      err = launch_ew(dtype, (const EwDesc*)(dbase + o_op[k]), (const int64_t*)(dbase + o_pre[k]),
+                      1, tiles[k], ew_grid(ctx, tiles[k], kChunk), kChunk, ctx->stream);
+      ctx->stats.ew_ops++;
+      ctx->stats.ew_ops_algo_bytes += op.bytes;
+    } else {
+      err = launch_gemm(dtype, true, (const GemmDesc*)(dbase + o_op[k]),
+                        (const int64_t*)(dbase + o_pre[k]), 1, tiles[k], gemm_grid(ctx, tiles[k]),
+                        ctx->stream);
+      ctx->stats.gemm_ops++;
+      ctx->stats.gemm_ops_algo_ops += 2 * op.macs;
+    }
+    ctx->stats.launches++;
+  }
+  ctx->stats.total_algo_ops += 2 * P.node_macs;
+  ctx->stats.leaf_tensors += P.leaves.size();
+  if (err == hipSuccess)
+    err = launch_epilogue(dtype, (const EpiDesc*)(dbase + o_epi), 1, dbase + o_res, ctx->stream);
+  ctx->stats.launches++;
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(ctx->staging.host, dbase + o_res, 4, hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t es = hipStreamSynchronize(ctx->stream);
+  ctx->pool.free(doff);
+  ctx->pool.free(arena_off);
+  if (err != hipSuccess || es != hipSuccess) {
+    set_error(std::string("plain slice: ") + hipGetErrorString(err != hipSuccess ? err : es));
+    return TBH_EDEVICE;
+  }
+  std::memcpy(result, ctx->staging.host, 4);
+  return 0;
+}
+
+}  // namespace tbh
+
+using namespace tbh;
+
+// ---------------------------------------------------------------- C ABI --
+extern "C" {
+
+const char* tbh_last_error(void) { return g_err.c_str(); }
+
+int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
+  if (!out) {
+    set_error("null output pointer");
+    return TBH_EINVAL;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) {
+    set_error("device " + std::to_string(device) + " not present (" + std::to_string(ndev) +
+              " devices)");
+    return TBH_EINVAL;
+  }
+  HIP_TRY(hipSetDevice(device));
+  auto ctx = std::make_unique<tbh_ctx>();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  ctx->num_cus = prop.multiProcessorCount;
+  size_t fr = 0, tot = 0;
+  HIP_TRY(hipMemGetInfo(&fr, &tot));
+  if (pool_bytes == 0) pool_bytes = (size_t)(fr * 0.8);
+  if (pool_bytes > fr) {
+    set_error("pool of " + std::to_string(pool_bytes) + " bytes exceeds free device memory (" +
+              std::to_string(fr) + ")");
+    return TBH_ENOMEM;
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, pool_bytes);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(ctx->stream);
+    set_error(std::string("hipMalloc pool: ") + hipGetErrorString(e));
+    return TBH_ENOMEM;
+  }
+  ctx->base = (char*)p;
+  ctx->pool.init(pool_bytes);
+  ctx->stats.pool_bytes = ctx->pool.capacity();
+  ctx->plan_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  *out = ctx.release();
+  return 0;
+}
+
+int tbh_serve_stop(tbh_ctx* ctx);
+
+int tbh_destroy(tbh_ctx* ctx) {
+  if (!ctx) return 0;
+  tbh_serve_stop(ctx);
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->base) (void)hipFree(ctx->base);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+static int contract_impl(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t n,
+                         void* results, tbh_mode mode, bool add_r) {
+  if (!ctx || (n && (!slices || !results))) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  if (mode != TBH_PLAIN && mode != TBH_BATCHED) {
+    set_error("unknown mode");
+    return TBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  std::vector<SliceOwned> owned(n);
+  std::vector<const SliceOwned*> ptrs(n);
+  for (size_t i = 0; i < n; i++) {
+    int rc = convert_slice(slices[i], dtype, false, &owned[i]);
+    if (rc) {
+      set_error("slice " + std::to_string(i) + ": " + tbh_last_error());
+      return rc;
+    }
+    ptrs[i] = &owned[i];
+  }
+  std::vector<SlicePlan> plans;
+  int rc = plan_all(ctx, ptrs, mode == TBH_BATCHED, &plans);
+  if (rc) return rc;
+  uint32_t* out = (uint32_t*)results;
+  if (mode == TBH_BATCHED) {
+    std::vector<const SlicePlan*> pp(n);
+    for (size_t i = 0; i < n; i++) pp[i] = &plans[i];
+    return execute_batched(ctx, dtype, ptrs, pp, add_r, out);
+  }
+  for (size_t i = 0; i < n; i++) {
+    rc = execute_plain(ctx, dtype, owned[i], plans[i], add_r, out + i);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int tbh_contract_slices(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t n,
+                        void* results, tbh_mode mode) {
+  return contract_impl(ctx, dtype, slices, n, results, mode, true);
+}
+
+int tbh_solve_slice(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slice, void* result,
+                    tbh_mode mode) {
+  return contract_impl(ctx, dtype, slice, 1, result, mode, false);
+}
+
+int tbh_contract_max(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t n,
+                     void* result, tbh_mode mode) {
+  if (n == 0) {
+    set_error("maximum over zero slices");
+    return TBH_EINVAL;
+  }
+  std::vector<uint32_t> r(n);
+  int rc = contract_impl(ctx, dtype, slices, n, r.data(), mode, true);
+  if (rc) return rc;
+  if (dtype == TBH_F32) {
+    float m = -INFINITY;
+    for (uint32_t b : r) {
+      float f;
+      std::memcpy(&f, &b, 4);
+      m = std::max(m, f);
+    }
+    *(float*)result = m;
+  } else {
+    int32_t m = INT32_MIN;
+    for (uint32_t b : r) m = std::max(m, (int32_t)b);
+    *(int32_t*)result = m;
+  }
+  return 0;
+}
+
+int tbh_maxplus_gemm(tbh_ctx* ctx, tbh_dtype dtype, int64_t M, int64_t N, int64_t K, int64_t batch,
+                     const void* A, int64_t lda, int64_t strideA, const void* B, int64_t ldb,
+                     int64_t strideB, void* C, int64_t ldc, int64_t strideC,
+                     tbh_ptr_kind ptr_kind) {
+  if (!ctx || dtype < 0 || dtype > 1) {
+    set_error("bad context or dtype");
+    return TBH_EINVAL;
+  }
+  if (M < 0 || N < 0 || K < 0 || batch < 0 || M > INT32_MAX || N > INT32_MAX || K > INT32_MAX ||
+      batch > INT32_MAX) {
+    set_error("sizes must be in [0, 2^31)");
+    return TBH_EINVAL;
+  }
+  if (lda < std::max<int64_t>(1, M) || ldb < std::max<int64_t>(1, K) ||
+      ldc < std::max<int64_t>(1, M)) {
+    set_error("leading dimension smaller than the matrix");
+    return TBH_EINVAL;
+  }
+  if (M == 0 || N == 0 || batch == 0) return 0;
+  if (!A || !B || !C) {
+    set_error("null matrix pointer");
+    return TBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int64_t ea = (batch - 1) * strideA + (K > 0 ? (K - 1) * lda + M : 0);
+  const int64_t eb = (batch - 1) * strideB + (N - 1) * ldb + K;
+  const int64_t ec = (batch - 1) * strideC + (N - 1) * ldc + M;
+  if (strideA < 0 || strideB < 0 || strideC < 0) {
+    set_error("negative batch stride");
+    return TBH_EINVAL;
+  }
+  std::vector<int64_t> offs;
+  char* dA = (char*)A;
+  char* dB = (char*)B;
+  char* dC = (char*)C;
+  auto release = [&]() {
+    for (int64_t o : offs) ctx->pool.free(o);
+  };
+  if (ptr_kind == TBH_HOST) {
+    for (auto [pp, ne] : {std::pair<char**, int64_t>{&dA, ea}, {&dB, eb}, {&dC, ec}}) {
+      int64_t o = ctx->pool.alloc((size_t)std::max<int64_t>(ne, 1) * 4);
+      if (o < 0) {
+        release();
+        set_error("GEMM operands do not fit the pool");
+        return TBH_ENOMEM;
+      }
+      offs.push_back(o);
+      *pp = ctx->base + o;
+    }
+    if (K > 0) HIP_TRY(hipMemcpyAsync(dA, A, 4 * ea, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dB, B, 4 * std::max<int64_t>(eb, 0), hipMemcpyHostToDevice, ctx->stream));
+  } else if (ptr_kind != TBH_DEVICE) {
+    set_error("unknown ptr_kind");
+    return TBH_EINVAL;
+  }
+  GemmDesc d;
+  std::memset(&d, 0, sizeof d);
+  d.a = (uint64_t)dA;
+  d.b = (uint64_t)dB;
+  d.c = (uint64_t)dC;
+  d.M = (int32_t)M;
+  d.N = (int32_t)N;
+  d.K = (int32_t)K;
+  d.batch = (int32_t)batch;
+  d.lda = lda;
+  d.ldb = ldb;
+  d.ldc = ldc;
+  d.sA = strideA;
+  d.sB = strideB;
+  d.sC = strideC;
+  d.tiles_m = (int32_t)((M + 127) / 128);
+  d.tiles_n = (int32_t)((N + 127) / 128);
+  const int64_t total = gemm_tiles(d);
+  const bool vec = M % 4 == 0 && K % 4 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
+                   strideA % 4 == 0 && strideB % 4 == 0 && strideC % 4 == 0 &&
+                   ((uint64_t)dA % 16 == 0) && ((uint64_t)dB % 16 == 0) && ((uint64_t)dC % 16 == 0);
+  int64_t doff = ctx->pool.alloc(sizeof d + 64);
+  if (doff < 0) {
+    release();
+    set_error("pool exhausted");
+    return TBH_ENOMEM;
+  }
+  offs.push_back(doff);
+  char* dd = ctx->base + doff;
+  int64_t pre[2] = {0, total};
+  int rc = ctx->staging.ensure(256);
+  if (rc) {
+    release();
+    return rc;
+  }
+  std::memcpy(ctx->staging.host, &d, sizeof d);
+  std::memcpy((char*)ctx->staging.host + sizeof d, pre, sizeof pre);
+  hipError_t e = hipMemcpyAsync(dd, ctx->staging.host, sizeof d + sizeof pre,
+                                hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = launch_gemm(dtype, vec, (const GemmDesc*)dd, (const int64_t*)(dd + sizeof d), 1, total,
+                    gemm_grid(ctx, total), ctx->stream);
+  ctx->stats.launches++;
+  if (e == hipSuccess && ptr_kind == TBH_HOST)
+    e = hipMemcpyAsync(C, dC, 4 * ec, hipMemcpyDeviceToHost, ctx->stream);
+  hipError_t es = hipStreamSynchronize(ctx->stream);
+  release();
+  if (e != hipSuccess || es != hipSuccess) {
+    set_error(std::string("maxplus_gemm: ") + hipGetErrorString(e != hipSuccess ? e : es));
+    return TBH_EDEVICE;
+  }
+  return 0;
+}
+
+static int view_slice(const tbh_slice* s, SliceIn* in) {
+  if (!s) {
+    set_error("null slice");
+    return TBH_EINVAL;
+  }
+  in->nv = s->nv;
+  in->nleaves = s->nv == 0 ? 0 : s->nleaves;
+  in->ix_ptr = s->ix_ptr;
+  in->ix = s->ix;
+  in->tree = s->tree;
+  in->r = s->r;
+  return 0;
+}
+
+int tbh_complexity(const tbh_slice* slice, double* tc, double* sc, double* rwc) {
+  SliceIn in;
+  if (view_slice(slice, &in)) return TBH_EINVAL;
+  std::string err;
+  if (slice_complexity(in, tc, sc, rwc, &err)) {
+    set_error(err);
+    return TBH_EINVAL;
+  }
+  return 0;
+}
+
+int tbh_peak_memory(const tbh_slice* slice, double* log2_elems) {
+  SliceIn in;
+  if (view_slice(slice, &in)) return TBH_EINVAL;
+  std::string err;
+  double all;
+  if (slice_peak_memory(in, log2_elems, &all, &err)) {
+    set_error(err);
+    return TBH_EINVAL;
+  }
+  return 0;
+}
+
+int tbh_all_memory(const tbh_slice* slice, double* log2_elems) {
+  SliceIn in;
+  if (view_slice(slice, &in)) return TBH_EINVAL;
+  std::string err;
+  double peak;
+  if (slice_peak_memory(in, &peak, log2_elems, &err)) {
+    set_error(err);
+    return TBH_EINVAL;
+  }
+  return 0;
+}
+
+int tbh_slice_footprint(const tbh_slice* slice, tbh_dtype dtype, size_t* bytes) {
+  SliceOwned o;
+  if (!slice || !bytes) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  int rc = convert_slice(*slice, dtype, false, &o);
+  if (rc) return rc;
+  SlicePlan P;
+  PlanOptions opt;
+  std::string err;
+  if (plan_slice(o.in, opt, &P, &err)) {
+    set_error(err);
+    return TBH_EINVAL;
+  }
+  *bytes = P.empty ? 0 : (size_t)P.arena_bytes;
+  return 0;
+}
+
+int tbh_get_stats(tbh_ctx* ctx, tbh_stats* out) {
+  if (!ctx || !out) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  *out = ctx->stats;
+  out->pool_bytes = ctx->pool.capacity();
+  out->pool_peak_bytes = ctx->pool.peak();
+  return 0;
+}
+
+int tbh_reset_stats(tbh_ctx* ctx) {
+  if (!ctx) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->stats = tbh_stats{};
+  ctx->pool.reset_peak();
+  return 0;
+}
+
+}  // extern "C"

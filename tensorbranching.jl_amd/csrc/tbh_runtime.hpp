@@ -1,0 +1,96 @@
+// tbh_runtime.hpp -- context, device pool and executors behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "tbh.h"
+#include "tbh_internal.hpp"
+#include "tbh_plan.hpp"
+
+namespace tbh {
+
+void set_error(const std::string& m);
+
+// launchers (tbh_kernels.hip)
+hipError_t launch_ew(int dtype, const EwDesc* ops, const int64_t* prefix, int nops, int64_t total,
+                     int grid, int chunk, hipStream_t s);
+hipError_t launch_gemm(int dtype, bool vec, const GemmDesc* ops, const int64_t* prefix, int nops,
+                       int64_t total, int grid, hipStream_t s);
+hipError_t launch_leaves(int dtype, const LeafDesc* leaves, int64_t n, hipStream_t s);
+hipError_t launch_epilogue(int dtype, const EpiDesc* e, int64_t n, void* out, hipStream_t s);
+
+// First-fit allocator over one device allocation (offsets, 256-byte units).
+class Pool {
+ public:
+  void init(size_t bytes);
+  int64_t alloc(size_t bytes);  // -1 when it does not fit
+  void free(int64_t off);
+  size_t capacity() const { return cap_; }
+  size_t used() const { return used_; }
+  size_t peak() const { return peak_; }
+  size_t largest_free() const;
+  void reset_peak() { peak_ = used_; }
+
+ private:
+  size_t cap_ = 0, used_ = 0, peak_ = 0;
+  std::map<int64_t, size_t> free_;   // offset -> size
+  std::map<int64_t, size_t> inuse_;  // offset -> size
+};
+
+struct Staging {  // pinned host buffer reused for descriptor uploads
+  void* host = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes);
+  ~Staging();
+};
+
+struct ServeState;
+
+}  // namespace tbh
+
+struct tbh_ctx {
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  char* base = nullptr;  // device pool
+  tbh::Pool pool;
+  tbh::Staging staging;
+  std::mutex mu;         // one executor call at a time
+  tbh_stats stats{};
+  int plan_threads = 8;
+  std::shared_ptr<tbh::ServeState> serve;  // created by tbh_serve_start
+};
+
+namespace tbh {
+
+// Converted input slice with its own storage.
+struct SliceOwned {
+  SliceIn in;
+  std::vector<uint32_t> wbits;
+  std::vector<int32_t> ix_ptr, ix, tree;
+  uint32_t rbits = 0;
+  bool empty_graph = false;  // nv == 0: result is r
+};
+
+int convert_slice(const tbh_slice& s, tbh_dtype dtype, bool copy, SliceOwned* out);
+
+// Executes plans for `slices`; results[i] (dtype bits) = root + r (add_r) or
+// root.  Used by the C ABI and by the serving scheduler.
+int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*>& slices,
+                    const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* results);
+int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SlicePlan& plan,
+                  bool add_r, uint32_t* result);
+int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool batched,
+             std::vector<SlicePlan>* plans);
+
+}  // namespace tbh

@@ -1,0 +1,62 @@
+"""Client of the continuous-batching MIS request scheduler (tbh_submit /
+tbh_poll / tbh_wait in include/tbh.h)."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+from .api import _dtype_code, _Marshal
+
+
+class Server:
+    """Requests (lists of SlicedBranch) are admitted by device-memory
+    footprint as resident slices finish; results are the MIS values."""
+
+    def __init__(self, backend, max_resident_slices=0):
+        self.be = backend
+        check(_lib.tbh().tbh_serve_start(backend.handle, int(max_resident_slices)))
+        self._open = True
+        self._dt = {}
+
+    def submit(self, branches, element_type=np.float32):
+        code, npdt = _dtype_code(element_type)
+        m = _Marshal(branches, npdt)
+        req = _lib.TbhRequest(code, m.ptr, m.n)
+        t = C.c_uint64()
+        check(_lib.tbh().tbh_submit(self.be.handle, C.byref(req), C.byref(t)))
+        self._dt[t.value] = npdt
+        return t.value
+
+    def poll(self, ticket):
+        npdt = self._dt[ticket]
+        out = np.zeros(1, npdt)
+        st = C.c_int()
+        t0, t1 = C.c_double(), C.c_double()
+        check(_lib.tbh().tbh_poll(self.be.handle, ticket, out.ctypes.data, C.byref(st),
+                                  C.byref(t0), C.byref(t1)))
+        return _lib.STATUS[st.value], out[0], t0.value, t1.value
+
+    def wait(self, ticket):
+        npdt = self._dt[ticket]
+        out = np.zeros(1, npdt)
+        st = C.c_int()
+        check(_lib.tbh().tbh_wait(self.be.handle, ticket, out.ctypes.data, C.byref(st)))
+        if _lib.STATUS[st.value] != "done":
+            raise _lib.TbhError(-1, f"request {ticket} {_lib.STATUS[st.value]}: "
+                                    f"{_lib.tbh().tbh_last_error().decode()}")
+        return out[0]
+
+    def cancel(self, ticket):
+        check(_lib.tbh().tbh_cancel(self.be.handle, ticket))
+
+    def close(self):
+        if self._open:
+            _lib.tbh().tbh_serve_stop(self.be.handle)
+            self._open = False
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,113 @@
+"""contract_slices on the GPU (PLAIN and BATCHED) against the oracle and
+independent MIS solvers, bit-exact."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _slices(tb, g, sc_target, weights=None, seed=0, trials=20):
+    return tb.slice_graph(g, weights, tb.ContractionTreeSlicer(sc_target=sc_target,
+                                                               order_trials=trials), seed=seed)
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_config1_unweighted(tb, orc, backend, mode):
+    """3-regular n=100 seed 0, sliced to sc_target 10: per-slice results equal
+    the oracle's and the maximum equals the exact MIS."""
+    g = tb.random_regular_graph(100, 3, seed=0)
+    br = _slices(tb, g, 10)
+    got = backend.contract_slices(br, np.float32, mode)
+    ref = orc.contract_slices(br, np.float32)
+    assert np.array_equal(got, ref)
+    assert int(got.max()) == orc.mis_exact(g)
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_weighted_int32(tb, orc, backend, mode):
+    g = tb.random_ksg(8, 8, 0.8, seed=3)
+    rng = np.random.default_rng(3)
+    w = rng.integers(1, 101, g.nv).astype(np.int32)
+    br = _slices(tb, g, 8, weights=w)
+    got = backend.contract_slices(br, np.int32, mode)
+    ref = orc.contract_slices(br, np.int32)
+    assert got.dtype == np.int32
+    assert np.array_equal(got, ref)
+    if g.nv <= 60:
+        assert int(got.max()) == int(orc.mis_bruteforce(g, w.astype(np.float64)))
+
+
+def test_int32_equals_float32(tb, backend):
+    g = tb.random_regular_graph(60, 3, seed=4)
+    br = _slices(tb, g, 8)
+    a = backend.contract_slices(br, np.float32)
+    b = backend.contract_slices(br, np.int32)
+    assert np.array_equal(a.astype(np.int64), b.astype(np.int64))
+
+
+def test_unsliced_whole_graph(tb, orc, backend):
+    g = tb.random_regular_graph(60, 3, seed=1)
+    code = tb.initialize_code(g, order_trials=20, seed=1)
+    br = [tb.SlicedBranch(tb.MISProblem(g), code, 0)]
+    for mode in ("batched", "plain"):
+        assert backend.solve_slice(br[0], np.float32, mode) == orc.mis_exact(g)
+
+
+def test_empty_and_tiny_slices(tb, orc, backend):
+    empty = tb.SlicedBranch(tb.MISProblem(tb.SimpleGraph(0)), None, 7)
+    one = tb.SlicedBranch(tb.MISProblem(tb.SimpleGraph(1), np.array([5], np.float32)),
+                          tb.CompressedEinsum([[0]], [], 0), 2)
+    g2 = tb.SimpleGraph(3, [[0, 1]])  # an edge plus an isolated vertex
+    code2 = tb.CompressedEinsum([[0], [1], [2], [0, 1]], [],
+                                tb.ContractionTree(tb.ContractionTree(0, 3),
+                                                   tb.ContractionTree(1, 2)))
+    two = tb.SlicedBranch(tb.MISProblem(g2), code2, 1)
+    br = [empty, one, two]
+    for mode in ("batched", "plain"):
+        got = backend.contract_slices(br, np.float32, mode)
+        assert got.tolist() == [7.0, 7.0, 3.0]
+        assert np.array_equal(got, orc.contract_slices(br, np.float32))
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_closed_forms(tb, backend, mode):
+    cases = []
+    for n in (5, 12, 31):  # cycles: floor(n/2)
+        cases.append((tb.SimpleGraph(n, [[i, (i + 1) % n] for i in range(n)]), n // 2))
+    for n in (6, 17):  # paths: ceil(n/2)
+        cases.append((tb.SimpleGraph(n, [[i, i + 1] for i in range(n - 1)]), (n + 1) // 2))
+    a, b = 4, 7  # complete bipartite: max(a, b)
+    cases.append((tb.SimpleGraph(a + b, [[i, a + j] for i in range(a) for j in range(b)]), 7))
+    pet = [[i, (i + 1) % 5] for i in range(5)] + [[i, i + 5] for i in range(5)] + \
+          [[5 + i, 5 + (i + 2) % 5] for i in range(5)]
+    cases.append((tb.SimpleGraph(10, pet), 4))  # Petersen
+    L = 7  # grid L x L: ceil(L^2 / 2)
+    grid = [[i * L + j, i * L + j + 1] for i in range(L) for j in range(L - 1)] + \
+           [[i * L + j, (i + 1) * L + j] for i in range(L - 1) for j in range(L)]
+    cases.append((tb.SimpleGraph(L * L, grid), (L * L + 1) // 2))
+    for g, expect in cases:
+        code = tb.initialize_code(g, order_trials=10)
+        br = tb.SlicedBranch(tb.MISProblem(g), code, 0)
+        assert backend.solve_slice(br, np.float32, mode) == expect
+
+
+def test_order_invariance(tb, orc, backend):
+    """Different trees over the same leaves give identical bits."""
+    g = tb.random_regular_graph(50, 3, seed=9)
+    vals = set()
+    for seed in range(4):
+        code = tb.initialize_code(g, order_trials=4 + seed, seed=seed)
+        br = tb.SlicedBranch(tb.MISProblem(g), code, 0)
+        vals.add(float(backend.solve_slice(br, np.float32)))
+    assert vals == {float(orc.mis_exact(g))}
+
+
+def test_slicing_invariant(tb, orc, backend):
+    """max_i(slice_i + r_i) equals the unsliced contraction."""
+    g = tb.random_regular_graph(80, 3, seed=5)
+    w = np.random.default_rng(5).integers(1, 50, g.nv).astype(np.float32)
+    br = _slices(tb, g, 9, weights=w)
+    assert len(br) > 1
+    whole = tb.SlicedBranch(tb.MISProblem(g, w), tb.initialize_code(g, 20), 0)
+    assert backend.contract_slices(br).max() == backend.solve_slice(whole) == \
+        orc.solve_slice(whole)

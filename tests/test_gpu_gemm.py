@@ -1,0 +1,58 @@
+"""Max-plus GEMM (tbh_maxplus_gemm, the TropicalGEMM mul! seam) on the GPU
+against the oracle, bit-exact."""
+import numpy as np
+import pytest
+
+from conftest import rand_tropical
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1, 1, 1), (3, 5, 7), (16, 16, 16), (64, 64, 64), (100, 37, 129), (128, 128, 16),
+          (256, 64, 512), (16, 4096, 16), (4096, 16, 64), (512, 512, 256), (130, 257, 3),
+          (2048, 256, 256)]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.int32])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_gemm_matches_oracle(backend, orc, dtype, shape):
+    M, N, K = shape
+    rng = np.random.default_rng(M * 7919 + N * 31 + K)
+    A = rand_tropical(rng, (M, K), dtype)
+    B = rand_tropical(rng, (K, N), dtype)
+    got = backend.maxplus_gemm(A, B)
+    ref = orc.maxplus_gemm(A, B)
+    assert got.dtype == ref.dtype
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.int32])
+def test_gemm_batched(backend, orc, dtype):
+    rng = np.random.default_rng(5)
+    A = rand_tropical(rng, (64, 32, 6), dtype)
+    B = rand_tropical(rng, (32, 96, 6), dtype)
+    assert np.array_equal(backend.maxplus_gemm(A, B), orc.maxplus_gemm(A, B))
+
+
+def test_gemm_large_identity(backend):
+    """A (x) I = A at M = N = K = 4096 (the tropical identity has 0 on the
+    diagonal and -inf elsewhere)."""
+    n = 4096
+    rng = np.random.default_rng(11)
+    A = rand_tropical(rng, (n, n), np.float32)
+    I = np.full((n, n), -np.inf, np.float32, order="F")
+    np.fill_diagonal(I, 0)
+    assert np.array_equal(backend.maxplus_gemm(A, I), A)
+    assert np.array_equal(backend.maxplus_gemm(I, A), A)
+
+
+def test_gemm_large_rank_one(backend):
+    """A[m,k] = u[m] + v[k] gives C[m,n] = u[m] + max_k(v[k] + B[k,n])."""
+    rng = np.random.default_rng(12)
+    M, N, K = 4096, 2048, 1024
+    u = rng.integers(0, 1000, M).astype(np.float32)
+    v = rng.integers(0, 1000, K).astype(np.float32)
+    B = rand_tropical(rng, (K, N), np.float32, neg_frac=0.0)
+    A = np.asfortranarray(u[:, None] + v[None, :])
+    C = backend.maxplus_gemm(A, B)
+    expect = u[:, None] + (v[:, None] + B).max(axis=0)[None, :]
+    assert np.array_equal(C, expect)

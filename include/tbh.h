@@ -151,6 +151,22 @@ int tbh_wait(tbh_ctx* ctx, uint64_t ticket, void* result, tbh_status* status);
 int tbh_cancel(tbh_ctx* ctx, uint64_t ticket);
 
 /* ---- introspection ---------------------------------------------------- */
+/* One planned step of a slice (no reference counterpart: used by tests and
+ * the bench to account algorithmic work per kernel). */
+typedef struct {
+  int32_t kind;             /* 0 EW step, 1 GEMM step                       */
+  int32_t level;            /* wave (BATCHED) or sequence index (PLAIN)     */
+  int32_t rank_a, rank_b, rank_c;  /* rank_b = -1: unary step               */
+  int32_t n_reduced;        /* EW: reduced labels                           */
+  int32_t M, N, K, batch;   /* GEMM extents                                 */
+  int32_t lanes_a, lanes_b; /* EW: contiguous A/B positions under lane bits */
+  double macs, bytes;       /* executed configurations, algorithmic bytes   */
+} tbh_step_info;
+/* Plans one slice and writes up to cap step records; returns the number of
+ * steps (>= 0) or a negative error code. */
+int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode,
+                   tbh_step_info* out, int cap);
+
 typedef struct {
   uint64_t launches;        /* kernel launches issued                     */
   uint64_t waves;           /* batched waves executed                     */
@@ -159,7 +175,16 @@ typedef struct {
   double gemm_ops_algo_ops; /* 2 * MACs of GEMM ops                        */
   double total_algo_ops;    /* 2 * MACs over all ops                       */
   size_t pool_bytes, pool_peak_bytes;
+  /* kernel time per kind from HIP events on the context's stream, filled
+   * while profiling is on (tbh_set_profiling): [0] small EW steps,
+   * [1] tiled EW steps, [2] GEMM steps */
+  double kind_ms[3];
+  double kind_bytes[3];     /* algorithmic bytes of the steps timed         */
+  double kind_macs[3];      /* executed (output x reduced) configurations   */
+  uint64_t kind_launches[3];
 } tbh_stats;
+/* Time every launch with HIP events (adds a little host overhead). */
+int tbh_set_profiling(tbh_ctx* ctx, int on);
 int tbh_get_stats(tbh_ctx* ctx, tbh_stats* out);
 int tbh_reset_stats(tbh_ctx* ctx);
 

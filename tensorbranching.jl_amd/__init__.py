@@ -12,7 +12,8 @@ from ._lib import TbhError, TBH_I32_NEG_INF  # noqa: F401
 from .api import (  # noqa: F401
     CompressedEinsum, ContractionComplexity, ContractionTree, HIPBackend, MISProblem,
     SimpleGraph, SlicedBranch, UnitWeight, code_complexity, complexity, contract_slices,
-    contraction_all_memory, contraction_peak_memory, default_backend, sc, slice_footprint,
+    contraction_all_memory, contraction_peak_memory, default_backend, plan_steps, sc,
+    slice_footprint,
     solve_slice, tc)
 from .workload import (  # noqa: F401
     ContractionTreeSlicer, dynamic_ob_mis, erdos_renyi, initialize_code, random_ksg,

@@ -14,6 +14,7 @@ TBH_F32, TBH_I32 = 0, 1
 TBH_PLAIN, TBH_BATCHED = 0, 1
 TBH_HOST, TBH_DEVICE = 0, 1
 TBH_I32_NEG_INF = -(1 << 30)
+TBH_EINVAL, TBH_ENOMEM, TBH_EDEVICE, TBH_EBUSY, TBH_ENOTFOUND = -1, -2, -3, -4, -5
 
 STATUS = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
@@ -33,7 +34,17 @@ class TbhStats(C.Structure):
                 ("gemm_ops", C.c_uint64), ("leaf_tensors", C.c_uint64),
                 ("ew_ops_algo_bytes", C.c_double), ("gemm_ops_algo_ops", C.c_double),
                 ("total_algo_ops", C.c_double), ("pool_bytes", C.c_size_t),
-                ("pool_peak_bytes", C.c_size_t)]
+                ("pool_peak_bytes", C.c_size_t), ("kind_ms", C.c_double * 3),
+                ("kind_bytes", C.c_double * 3), ("kind_macs", C.c_double * 3),
+                ("kind_launches", C.c_uint64 * 3)]
+
+
+class TbhStepInfo(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("level", C.c_int32), ("rank_a", C.c_int32),
+                ("rank_b", C.c_int32), ("rank_c", C.c_int32), ("n_reduced", C.c_int32),
+                ("M", C.c_int32), ("N", C.c_int32), ("K", C.c_int32), ("batch", C.c_int32),
+                ("lanes_a", C.c_int32), ("lanes_b", C.c_int32), ("macs", C.c_double),
+                ("bytes", C.c_double)]
 
 
 class TbwSliceView(C.Structure):
@@ -70,7 +81,10 @@ TBH_SIGNATURES = {
     "tbh_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_int)]),
     "tbh_cancel": (C.c_int, [C.c_void_p, C.c_uint64]),
     "tbh_get_stats": (C.c_int, [C.c_void_p, C.POINTER(TbhStats)]),
+    "tbh_plan_steps": (C.c_int, [C.POINTER(TbhSlice), C.c_int, C.c_int, C.POINTER(TbhStepInfo),
+                                 C.c_int]),
     "tbh_reset_stats": (C.c_int, [C.c_void_p]),
+    "tbh_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
 TBW_SIGNATURES = {

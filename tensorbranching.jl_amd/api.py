@@ -307,9 +307,18 @@ class HIPBackend:
         return Cm
 
     def stats(self):
+        """Counters; kind_* lists are [small EW, tiled EW, GEMM] (times from
+        HIP events while profiling is on)."""
         s = _lib.TbhStats()
         check(_lib.tbh().tbh_get_stats(self.handle, C.byref(s)))
-        return {k: getattr(s, k) for k, _ in s._fields_}
+        out = {}
+        for k, _ in s._fields_:
+            v = getattr(s, k)
+            out[k] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+    def set_profiling(self, on=True):
+        check(_lib.tbh().tbh_set_profiling(self.handle, 1 if on else 0))
 
     def reset_stats(self):
         check(_lib.tbh().tbh_reset_stats(self.handle))
@@ -422,6 +431,18 @@ def contraction_peak_memory(code, nv=None):
 def contraction_all_memory(code, nv=None):
     """log2 of all elements allocated by internal nodes (src/utils.jl:221-229)."""
     return _mem(_lib.tbh().tbh_all_memory, code, nv)
+
+
+def plan_steps(branch, element_type=np.float32, mode="batched"):
+    """The steps the executor runs for one slice, as a list of dicts."""
+    code, npdt = _dtype_code(element_type)
+    m = _Marshal([branch], npdt)
+    n = _lib.tbh().tbh_plan_steps(m.ptr, code, _mode(mode), None, 0)
+    if n < 0:
+        check(n)
+    arr = (_lib.TbhStepInfo * max(1, n))()
+    check(min(0, _lib.tbh().tbh_plan_steps(m.ptr, code, _mode(mode), arr, n)))
+    return [{k: getattr(arr[i], k) for k, _ in _lib.TbhStepInfo._fields_} for i in range(n)]
 
 
 def slice_footprint(branch, element_type=np.float32):

@@ -9,26 +9,38 @@ namespace tbh {
 constexpr int32_t kI32NegInf = -(1 << 30);
 constexpr int32_t kI32NegThresh = -(1 << 29);
 constexpr int kMaxRank = 30;     // largest tensor rank (2^30 entries)
-constexpr int kMaxReduce = 15;   // reduced labels of one EW step
-constexpr int kWavesPerBlock = 4;
-constexpr int kEwBlock = 64 * kWavesPerBlock;
+constexpr int kMaxReduce = 12;   // reduced labels of one EW step
+constexpr int kEwBlock = 256;    // threads per EW workgroup
+constexpr int kTileBits = 12;    // a tile has at most 2^12 outputs
+constexpr int kSubBits = 12;     // an LDS sub-tensor has at most 2^12 entries
+constexpr int kLdsElems = (1 << kSubBits) + ((1 << kSubBits) >> 5);  // padded
 
-// One EW step: C[c] = max over r of A[dep_a(c) + rdep_a(r)] + B[...], where
-// dep_a(c) = sum of 2^pa[i] over the set bits i of c (pa[i]: position in A
-// of the label at C position i, -1 if A lacks it) and likewise for the
-// reduced labels.  Lanes take output bits 0..5, a lane's 2^ept elements the next
-// bits, and the wave tile index the rest.
+// One EW step C = max over the reduced labels of A + B, tiled through LDS.
+// The output labels split into tile labels (nT bits, ordered by C position)
+// and outer labels (nO bits, one tile per configuration).  A tile first loads
+// the sub-tensors of A and B it needs -- their tile and reduced labels, nSA /
+// nSB bits in ascending operand position, so consecutive threads read
+// consecutive addresses along the operand's lowest positions -- into LDS,
+// then each of its outputs maxes A + B over the 2^nR reduced configurations
+// and is written back contiguously along C's lowest positions.
+// All positions are label positions (stride 2^p); -1 marks an absent label.
 struct EwDesc {
-  uint64_t a, b, c;  // device addresses; b == 0: unary step (B = tropical 1)
-  int32_t nP;        // output rank
-  int32_t nR;        // reduced rank
-  int32_t ept;       // log2 elements per lane (0..4)
-  int32_t tiles;     // wave tiles of this step
-  int8_t pa[32], pb[32];    // output bit i -> position in A / B (-1: absent)
-  int8_t ra[16], rb[16];    // reduced bit i -> position in A / B (-1: absent)
+  uint64_t a, b, c;        // device addresses; b == 0: unary step (B = tropical 1)
+  int32_t nT, nO, nR;      // tile, outer and reduced bits
+  int32_t nSA, nSB;        // sub-tensor bits of A and B
+  int32_t tiles;           // 2^nO
+  int8_t tC[kTileBits];    // tile bit -> C position
+  int8_t tA[kTileBits];    // tile bit -> bit of A's sub-tensor
+  int8_t tB[kTileBits];
+  int8_t rA[kMaxReduce];   // reduced bit -> bit of A's / B's sub-tensor
+  int8_t rB[kMaxReduce];
+  int8_t sA[kSubBits];     // sub-tensor bit -> A position
+  int8_t sB[kSubBits];
+  int8_t oC[32], oA[32], oB[32];  // outer bit -> C / A / B position
 };
 
 __host__ __device__ inline uint32_t pos_stride(int8_t p) { return p >= 0 ? (1u << p) : 0u; }
+__host__ __device__ inline uint32_t lds_pad(uint32_t i) { return i + (i >> 5); }
 
 // One batched max-plus GEMM step (column-major, element strides).
 struct GemmDesc {

@@ -50,162 +50,299 @@ template <>
 __device__ __forceinline__ int32_t pad_zero<int32_t>() { return kI32NegInf; }
 
 // ---------------------------------------------------------------- K2 -----
-// Reduced-label loop: the lowest NLO (<= 2) reduced bits are unrolled, the
-// rest walk a Gray code so each step changes one stride.
-template <typename T, int EPT, int NLO, bool HASB>
-__device__ __forceinline__ void ew_tile(const T* __restrict__ A, const T* __restrict__ B,
-                                        T* __restrict__ C, uint32_t baseA, uint32_t baseB,
-                                        const uint32_t (&jA)[EPT], const uint32_t (&jB)[EPT],
-                                        const EwDesc* __restrict__ d, int nR, uint32_t cbase,
-                                        int lane, bool active) {
-  T acc[EPT];
+// The 4 high bits (8..11) of a per-thread index: strides of those bits, and
+// the sum over the bits set in a compile-time m.
+__device__ __forceinline__ void hi_strides(uint32_t (&h)[4], const int8_t* pos) {
 #pragma unroll
-  for (int j = 0; j < EPT; j++) acc[j] = Trop<T>::zero();
-  uint32_t loA[4] = {0, 0, 0, 0}, loB[4] = {0, 0, 0, 0};
-  if (NLO >= 1) {
-    loA[1] = pos_stride(d->ra[0]);
-    loB[1] = pos_stride(d->rb[0]);
+  for (int b = 0; b < 4; b++) h[b] = pos_stride(pos[8 + b]);
+}
+template <int M>
+__device__ __forceinline__ uint32_t hi_at(const uint32_t (&h)[4]) {
+  return ((M & 1) ? h[0] : 0u) + ((M & 2) ? h[1] : 0u) + ((M & 4) ? h[2] : 0u) +
+         ((M & 8) ? h[3] : 0u);
+}
+__device__ __forceinline__ uint32_t hi_at(const uint32_t (&h)[4], int m) {
+  return ((m & 1) ? h[0] : 0u) + ((m & 2) ? h[1] : 0u) + ((m & 4) ? h[2] : 0u) +
+         ((m & 8) ? h[3] : 0u);
+}
+
+// the low 8 bits of the thread index mapped through pos[0..7]
+__device__ __forceinline__ uint32_t lo_dep(uint32_t tid, const int8_t* pos) {
+  uint32_t v = 0;
+#pragma unroll
+  for (int b = 0; b < 8; b++)
+    if ((tid >> b) & 1) v += pos_stride(pos[b]);
+  return v;
+}
+
+struct EwThread {  // per-step constants of one thread
+  uint32_t loSA, loSB, loTA, loTB, loTC;
+  uint32_t hSA[4], hSB[4], hTA[4], hTB[4], hTC[4];
+};
+
+__device__ __forceinline__ void ew_thread_setup(const EwDesc* __restrict__ d, EwThread& e) {
+  const uint32_t tid = threadIdx.x;
+  e.loSA = lo_dep(tid, d->sA);
+  e.loSB = lo_dep(tid, d->sB);
+  e.loTA = lo_dep(tid, d->tA);
+  e.loTB = lo_dep(tid, d->tB);
+  e.loTC = lo_dep(tid, d->tC);
+  hi_strides(e.hSA, d->sA);
+  hi_strides(e.hSB, d->sB);
+  hi_strides(e.hTA, d->tA);
+  hi_strides(e.hTB, d->tB);
+  hi_strides(e.hTC, d->tC);
+}
+
+// One tile: outer configuration o of step d.
+template <typename T, bool HASB>
+__device__ __forceinline__ void ew_tile(const EwDesc* __restrict__ d, uint32_t o,
+                                        const EwThread& e, T* __restrict__ lA,
+                                        T* __restrict__ lB) {
+  const int tid = threadIdx.x;
+  uint32_t offA = 0, offB = 0, offC = 0;
+  const int nO = d->nO;
+#pragma unroll
+  for (int k = 0; k < 32; k++)
+    if (k < nO && ((o >> k) & 1)) {
+      offA += pos_stride(d->oA[k]);
+      offB += pos_stride(d->oB[k]);
+      offC += pos_stride(d->oC[k]);
+    }
+  const T* __restrict__ A = reinterpret_cast<const T*>(d->a) + offA;
+  const T* __restrict__ B = reinterpret_cast<const T*>(d->b) + offB;
+  T* __restrict__ C = reinterpret_cast<T*>(d->c) + offC;
+  // 1. the tile's sub-tensors of A and B into LDS (contiguous reads)
+  {
+    const int nA = 1 << d->nSA;
+    T v[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      if (tid + 256 * m < nA) v[m] = A[e.loSA + hi_at(e.hSA, m)];
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      if (tid + 256 * m < nA) lA[lds_pad(tid + 256 * m)] = v[m];
   }
-  if (NLO >= 2) {
-    loA[2] = pos_stride(d->ra[1]);
-    loB[2] = pos_stride(d->rb[1]);
-    loA[3] = loA[1] + loA[2];
-    loB[3] = loB[1] + loB[2];
+  if (HASB) {
+    const int nB = 1 << d->nSB;
+    T v[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      if (tid + 256 * m < nB) v[m] = B[e.loSB + hi_at(e.hSB, m)];
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      if (tid + 256 * m < nB) lB[lds_pad(tid + 256 * m)] = v[m];
   }
-  const int nhi = nR - NLO;
-  const uint32_t nh = 1u << nhi;
-  uint32_t hA = 0, hB = 0;
+  __syncthreads();
+  // 2. max over the reduced configurations: the lowest (up to 3) reduced
+  //    bits are unrolled with their strides in registers, taken in pairs
+  //    (add + max3); the rest walk a Gray code, one stride change each
+  const int nOut = 1 << d->nT;
+  T acc[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) acc[m] = Trop<T>::zero();
+  const int nR = d->nR;
+  const int nlo = nR < 3 ? nR : 3;
+  uint32_t lsA[4] = {0, 0, 0, 0}, lsB[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int b = 0; b < 3; b++)
+    if (b < nlo) {
+      lsA[b] = pos_stride(d->rA[b]);
+      lsB[b] = pos_stride(d->rB[b]);
+    }
+  const int nq = 1 << nlo;
+  const uint32_t nh = 1u << (nR - nlo);
+  uint32_t rA = 0, rB = 0;
   for (uint32_t h = 0; h < nh; h++) {
     if (h) {
       const int i = __builtin_ctz(h);
       const uint32_t g = h ^ (h >> 1);
-      const uint32_t sa = pos_stride(d->ra[NLO + i]), sb = pos_stride(d->rb[NLO + i]);
+      const uint32_t sa = pos_stride(d->rA[nlo + i]), sb = pos_stride(d->rB[nlo + i]);
       if ((g >> i) & 1) {
-        hA += sa;
-        hB += sb;
+        rA += sa;
+        rB += sb;
       } else {
-        hA -= sa;
-        hB -= sb;
+        rA -= sa;
+        rB -= sb;
       }
     }
 #pragma unroll
-    for (int q = 0; q < (1 << NLO); q += (NLO ? 2 : 1)) {
+    for (int q = 0; q < 8; q += 2) {
+      if (q >= nq) break;
+      const uint32_t qa0 = rA + hi_at(lsA, q), qb0 = rB + hi_at(lsB, q);
+      const uint32_t qa1 = rA + hi_at(lsA, q + 1), qb1 = rB + hi_at(lsB, q + 1);
 #pragma unroll
-      for (int j = 0; j < EPT; j++) {
-        const uint32_t oa = baseA + jA[j] + hA;
-        const uint32_t ob = baseB + jB[j] + hB;
-        T x0 = A[oa + loA[q]];
-        if (HASB) x0 = Trop<T>::mul(x0, B[ob + loB[q]]);
-        if (NLO) {
-          T x1 = A[oa + loA[q + 1]];
-          if (HASB) x1 = Trop<T>::mul(x1, B[ob + loB[q + 1]]);
-          acc[j] = Trop<T>::add(acc[j], Trop<T>::add(x0, x1));
-        } else {
-          acc[j] = Trop<T>::add(acc[j], x0);
+      for (int m = 0; m < 16; m++) {
+        if (tid + 256 * m < nOut) {
+          const uint32_t ia = e.loTA + hi_at(e.hTA, m), ib = e.loTB + hi_at(e.hTB, m);
+          T x0 = lA[lds_pad(ia + qa0)];
+          if (HASB) x0 = Trop<T>::mul(x0, lB[lds_pad(ib + qb0)]);
+          if (nq > 1) {
+            T x1 = lA[lds_pad(ia + qa1)];
+            if (HASB) x1 = Trop<T>::mul(x1, lB[lds_pad(ib + qb1)]);
+            acc[m] = Trop<T>::add(acc[m], Trop<T>::add(x0, x1));
+          } else {
+            acc[m] = Trop<T>::add(acc[m], x0);
+          }
         }
       }
     }
   }
-  if (active) {
+  // 3. contiguous writes along C's lowest positions
 #pragma unroll
-    for (int j = 0; j < EPT; j++) C[cbase + ((uint32_t)j << 6) + lane] = Trop<T>::canon(acc[j]);
+  for (int m = 0; m < 16; m++)
+    if (tid + 256 * m < nOut) C[e.loTC + hi_at(e.hTC, m)] = Trop<T>::canon(acc[m]);
+  __syncthreads();
+}
+
+// Direct steps: wave-level tiles of at most 2^10 outputs (lane = tile bits
+// 0..5, a lane's 16 elements = bits 6..9), operands read straight from
+// global memory -- used when every operand is either small (cache resident)
+// or aligned, i.e. its lowest positions are tile lane bits or reduced
+// labels, so the loads are coalesced without LDS and without barriers.  The
+// operand position of tile bit k is sA[tA[k]] (sB[tB[k]]), of reduced bit i
+// sA[rA[i]], of outer bit k oA[k].
+__device__ __forceinline__ uint32_t sub_stride(const int8_t* s, int8_t t) {
+  return t >= 0 ? pos_stride(s[t]) : 0u;
+}
+
+struct EwLane {  // per-step constants of one lane
+  uint32_t la, lb, lc;
+  uint32_t hA[4], hB[4], hC[4];
+};
+
+__device__ __forceinline__ void ew_lane_setup(const EwDesc* __restrict__ d, int lane, EwLane& L) {
+  const int nT = d->nT;
+  L.la = L.lb = L.lc = 0;
+#pragma unroll
+  for (int k = 0; k < 6; k++)
+    if (k < nT && ((lane >> k) & 1)) {
+      L.la += sub_stride(d->sA, d->tA[k]);
+      L.lb += sub_stride(d->sB, d->tB[k]);
+      L.lc += pos_stride(d->tC[k]);
+    }
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    L.hA[b] = 6 + b < nT ? sub_stride(d->sA, d->tA[6 + b]) : 0u;
+    L.hB[b] = 6 + b < nT ? sub_stride(d->sB, d->tB[6 + b]) : 0u;
+    L.hC[b] = 6 + b < nT ? pos_stride(d->tC[6 + b]) : 0u;
   }
 }
 
-template <typename T, int EPT, int NLO, bool HASB>
-__device__ void ew_range(const EwDesc* __restrict__ d, int64_t l0, int64_t l1, int lane) {
-  const int nP = d->nP, nR = d->nR;
-  const T* A = reinterpret_cast<const T*>(d->a);
-  const T* B = reinterpret_cast<const T*>(d->b);
-  T* C = reinterpret_cast<T*>(d->c);
-  constexpr int E = EPT == 1 ? 0 : EPT == 2 ? 1 : EPT == 4 ? 2 : EPT == 8 ? 3 : 4;
-  uint32_t laneA = 0, laneB = 0;
+// outer offsets of tile o (sum of the strides of its set bits)
+struct EwOuter {
+  uint32_t a, b, c;
+};
+__device__ __forceinline__ EwOuter outer_offsets(const EwDesc* __restrict__ d, uint32_t o) {
+  EwOuter r{0, 0, 0};
+  const int nO = d->nO;
 #pragma unroll
-  for (int i = 0; i < 6; i++)
-    if (i < nP && ((lane >> i) & 1)) {
-      laneA += pos_stride(d->pa[i]);
-      laneB += pos_stride(d->pb[i]);
+  for (int k = 0; k < 32; k++)
+    if (k < nO && ((o >> k) & 1)) {
+      r.a += pos_stride(d->oA[k]);
+      r.b += pos_stride(d->oB[k]);
+      r.c += pos_stride(d->oC[k]);
     }
-  const bool active = nP >= 6 || lane < (1 << nP);
-  uint32_t jA[EPT], jB[EPT];
-  jA[0] = 0;
-  jB[0] = 0;
-#pragma unroll
-  for (int b = 0; b < E; b++) {
-    const uint32_t sa = pos_stride(d->pa[6 + b]), sb = pos_stride(d->pb[6 + b]);
-#pragma unroll
-    for (int j = 0; j < (1 << b); j++) {
-      jA[j + (1 << b)] = jA[j] + sa;
-      jB[j + (1 << b)] = jB[j] + sb;
+  return r;
+}
+// from tile o-1 to tile o: only the bits that flip change
+__device__ __forceinline__ void outer_step(const EwDesc* __restrict__ d, uint32_t o, EwOuter& r) {
+  uint32_t flip = o ^ (o - 1);
+  while (flip) {
+    const int k = __builtin_ctz(flip);
+    flip &= flip - 1;
+    const uint32_t sa = pos_stride(d->oA[k]), sb = pos_stride(d->oB[k]), sc = pos_stride(d->oC[k]);
+    if ((o >> k) & 1) {
+      r.a += sa;
+      r.b += sb;
+      r.c += sc;
+    } else {
+      r.a -= sa;
+      r.b -= sb;
+      r.c -= sc;
     }
   }
-  const int tb = 6 + E;
-  uint32_t tA = 0, tB = 0;
-  for (int i = tb; i < nP; i++)
-    if ((l0 >> (i - tb)) & 1) {
-      tA += pos_stride(d->pa[i]);
-      tB += pos_stride(d->pb[i]);
-    }
-  for (int64_t l = l0; l < l1; l++) {
-    if (l != l0) {
-      uint64_t flip = (uint64_t)l ^ (uint64_t)(l - 1);
-      while (flip) {
-        const int i = __builtin_ctzll(flip);
-        flip &= flip - 1;
-        const uint32_t sa = pos_stride(d->pa[tb + i]), sb = pos_stride(d->pb[tb + i]);
-        if (((uint64_t)l >> i) & 1) {
-          tA += sa;
-          tB += sb;
-        } else {
-          tA -= sa;
-          tB -= sb;
+}
+
+template <typename T, bool HASB>
+__device__ __forceinline__ void ew_direct_tile(const EwDesc* __restrict__ d, const EwLane& L,
+                                               const EwOuter& off, int lane) {
+  const T* __restrict__ A = reinterpret_cast<const T*>(d->a) + off.a;
+  const T* __restrict__ B = reinterpret_cast<const T*>(d->b) + off.b;
+  T* __restrict__ C = reinterpret_cast<T*>(d->c) + off.c;
+  const int nOut = 1 << d->nT, nR = d->nR;
+  T acc[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) acc[m] = Trop<T>::zero();
+  if (nR <= 1) {
+    // the common spine step: both reduced configurations' loads in flight
+    // before any is consumed
+    const uint32_t sA = nR ? sub_stride(d->sA, d->rA[0]) : 0u;
+    const uint32_t sB = nR ? sub_stride(d->sB, d->rB[0]) : 0u;
+    T x0[16], x1[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      if (lane + 64 * m < nOut) {
+        const uint32_t ia = L.la + hi_at(L.hA, m), ib = L.lb + hi_at(L.hB, m);
+        x0[m] = A[ia];
+        if (nR) x1[m] = A[ia + sA];
+        if (HASB) {
+          x0[m] = Trop<T>::mul(x0[m], B[ib]);
+          if (nR) x1[m] = Trop<T>::mul(x1[m], B[ib + sB]);
         }
       }
     }
-    ew_tile<T, EPT, NLO, HASB>(A, B, C, tA + laneA, tB + laneB, jA, jB, d, nR,
-                               (uint32_t)(l << tb), lane, active);
-  }
-}
-
-template <typename T, int EPT>
-__device__ __forceinline__ void ew_dispatch_nlo(const EwDesc* d, int64_t l0, int64_t l1,
-                                                int lane) {
-  const int nR = d->nR;
-  const bool hasB = d->b != 0;
-  if (nR == 0) {
-    if (hasB)
-      ew_range<T, EPT, 0, true>(d, l0, l1, lane);
-    else
-      ew_range<T, EPT, 0, false>(d, l0, l1, lane);
-  } else if (nR == 1) {
-    if (hasB)
-      ew_range<T, EPT, 1, true>(d, l0, l1, lane);
-    else
-      ew_range<T, EPT, 1, false>(d, l0, l1, lane);
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      if (lane + 64 * m < nOut) acc[m] = nR ? Trop<T>::add(x0[m], x1[m]) : x0[m];
   } else {
-    if (hasB)
-      ew_range<T, EPT, 2, true>(d, l0, l1, lane);
-    else
-      ew_range<T, EPT, 2, false>(d, l0, l1, lane);
+    uint32_t rA = 0, rB = 0;
+    const uint32_t nr = 1u << nR;
+    for (uint32_t r = 0; r < nr; r++) {
+      if (r) {
+        const int i = __builtin_ctz(r);
+        const uint32_t g = r ^ (r >> 1);
+        const uint32_t sa = sub_stride(d->sA, d->rA[i]), sb = sub_stride(d->sB, d->rB[i]);
+        if ((g >> i) & 1) {
+          rA += sa;
+          rB += sb;
+        } else {
+          rA -= sa;
+          rB -= sb;
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        if (lane + 64 * m < nOut) {
+          T x = A[L.la + hi_at(L.hA, m) + rA];
+          if (HASB) x = Trop<T>::mul(x, B[L.lb + hi_at(L.hB, m) + rB]);
+          acc[m] = Trop<T>::add(acc[m], x);
+        }
+      }
+    }
   }
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    if (lane + 64 * m < nOut) C[L.lc + hi_at(L.hC, m)] = Trop<T>::canon(acc[m]);
 }
 
-// Persistent: every wave walks contiguous chunks of the global tile space;
-// prefix[i] is the first tile of step i (prefix[nops] = total).
+// Persistent over wave tiles: prefix[i] is the first tile of step i.
 template <typename T>
-__global__ __launch_bounds__(kEwBlock) void tbh_ew_kernel(const EwDesc* __restrict__ ops,
-                                                          const int64_t* __restrict__ prefix,
-                                                          int nops, int64_t total, int chunk) {
+__global__ __launch_bounds__(256) void tbh_ew_direct_kernel(const EwDesc* __restrict__ ops,
+                                                            const int64_t* __restrict__ prefix,
+                                                            int nops, int64_t total, int chunk) {
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t nwaves = (int64_t)gridDim.x * kWavesPerBlock;
-  const int64_t gw = (int64_t)blockIdx.x * kWavesPerBlock + wid;
-  int op = 0;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int64_t gw = (int64_t)blockIdx.x * 4 + wid;
+  EwLane L;
+  EwOuter off{0, 0, 0};
+  int op = -1;
   int64_t ob = 0, oe = 0;
-  for (int64_t c0 = gw * chunk; c0 < total; c0 += nwaves * chunk) {
+  for (int64_t c0 = gw * chunk; c0 < total; c0 += nw * chunk) {
     const int64_t c1 = c0 + chunk < total ? c0 + chunk : total;
-    int64_t t = c0;
-    while (t < c1) {
+    for (int64_t t = c0; t < c1; t++) {
+      const bool fresh = t < ob || t >= oe || t == c0;
       if (t < ob || t >= oe) {
         int lo = 0, hi = nops;
         while (hi - lo > 1) {
@@ -218,17 +355,56 @@ __global__ __launch_bounds__(kEwBlock) void tbh_ew_kernel(const EwDesc* __restri
         op = lo;
         ob = prefix[op];
         oe = prefix[op + 1];
+        ew_lane_setup(ops + op, lane, L);
       }
-      const int64_t te = c1 < oe ? c1 : oe;
       const EwDesc* d = ops + op;
-      switch (d->ept) {
-        case 0: ew_dispatch_nlo<T, 1>(d, t - ob, te - ob, lane); break;
-        case 1: ew_dispatch_nlo<T, 2>(d, t - ob, te - ob, lane); break;
-        case 2: ew_dispatch_nlo<T, 4>(d, t - ob, te - ob, lane); break;
-        case 3: ew_dispatch_nlo<T, 8>(d, t - ob, te - ob, lane); break;
-        default: ew_dispatch_nlo<T, 16>(d, t - ob, te - ob, lane); break;
+      const uint32_t o = (uint32_t)(t - ob);
+      if (fresh)
+        off = outer_offsets(d, o);
+      else
+        outer_step(d, o, off);
+      if (d->b != 0)
+        ew_direct_tile<T, true>(d, L, off, lane);
+      else
+        ew_direct_tile<T, false>(d, L, off, lane);
+    }
+  }
+}
+
+// Persistent: each workgroup walks chunks of the global tile space; prefix[i]
+// is the first tile of step i (prefix[nops] = total).
+template <typename T>
+__global__ __launch_bounds__(kEwBlock) void tbh_ew_kernel(const EwDesc* __restrict__ ops,
+                                                          const int64_t* __restrict__ prefix,
+                                                          int nops, int64_t total, int chunk) {
+  __shared__ T lA[kLdsElems];
+  __shared__ T lB[kLdsElems];
+  EwThread e;
+  int op = -1;
+  int64_t ob = 0, oe = 0;
+  for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < total; c0 += (int64_t)gridDim.x * chunk) {
+    const int64_t c1 = c0 + chunk < total ? c0 + chunk : total;
+    for (int64_t t = c0; t < c1; t++) {
+      if (t < ob || t >= oe) {
+        int lo = 0, hi = nops;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (prefix[mid] <= t)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        op = lo;
+        ob = prefix[op];
+        oe = prefix[op + 1];
+        ew_thread_setup(ops + op, e);
       }
-      t = te;
+      const EwDesc* d = ops + op;
+      const uint32_t o = (uint32_t)(t - ob);
+      if (d->b != 0)
+        ew_tile<T, true>(d, o, e, lA, lB);
+      else
+        ew_tile<T, false>(d, o, e, lA, lB);
     }
   }
 }
@@ -461,6 +637,16 @@ hipError_t launch_ew(int dtype, const EwDesc* ops, const int64_t* prefix, int no
     tbh_ew_kernel<float><<<grid, kEwBlock, 0, s>>>(ops, prefix, nops, total, chunk);
   else
     tbh_ew_kernel<int32_t><<<grid, kEwBlock, 0, s>>>(ops, prefix, nops, total, chunk);
+  return hipGetLastError();
+}
+
+hipError_t launch_ew_direct(int dtype, const EwDesc* ops, const int64_t* prefix, int nops,
+                            int64_t total, int grid, int chunk, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  if (dtype == 0)
+    tbh_ew_direct_kernel<float><<<grid, 256, 0, s>>>(ops, prefix, nops, total, chunk);
+  else
+    tbh_ew_direct_kernel<int32_t><<<grid, 256, 0, s>>>(ops, prefix, nops, total, chunk);
   return hipGetLastError();
 }
 

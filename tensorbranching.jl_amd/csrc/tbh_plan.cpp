@@ -130,25 +130,22 @@ struct Builder {
     op.c = c;
     op.nP = (int)LC.size();
     if (op.nP > kMaxRank) return fail(err, "tensor rank above 30");
-    std::memset(op.pa, -1, sizeof op.pa);
-    std::memset(op.pb, -1, sizeof op.pb);
-    std::memset(op.ra, -1, sizeof op.ra);
-    std::memset(op.rb, -1, sizeof op.rb);
-    for (int i = 0; i < op.nP; i++) {
-      op.pa[i] = (int8_t)pos_in(LA, LC[i]);
-      op.pb[i] = (int8_t)pos_in(LB, LC[i]);
-      if (op.pa[i] < 0 && op.pb[i] < 0) return fail(err, "output label missing from inputs");
-    }
+    for (int32_t l : LC)
+      if (pos_in(LA, l) < 0 && pos_in(LB, l) < 0)
+        return fail(err, "output label missing from inputs");
     // reduced labels, ordered by their position in A then B
     std::vector<int32_t> R;
     for (int32_t l : LA)
       if (pos_in(LC, l) < 0) R.push_back(l);
     for (int32_t l : LB)
       if (pos_in(LC, l) < 0 && pos_in(LA, l) < 0) R.push_back(l);
-    if (R.size() > 8 && op.nP < 20) {
-      // split-K: keep the last s reduced labels in a temporary (more lanes,
-      // shorter loops), then reduce them in a second step
-      const int s_keep = std::min((int)R.size() - 4, 20 - op.nP);
+    if ((R.size() > 8 && op.nP < 20) || R.size() > (size_t)kMaxReduce) {
+      // split-K: keep the last s reduced labels in a temporary (more
+      // outputs, shorter loops), then reduce them in a second step
+      const int s_keep = R.size() > (size_t)kMaxReduce && op.nP >= 20
+                             ? (int)R.size() - kMaxReduce
+                             : std::min((int)R.size() - 4, 20 - op.nP);
+      if (op.nP + s_keep > 28) return fail(err, "EW step too large to split");
       std::vector<int32_t> tl = LC;
       tl.insert(tl.end(), R.end() - s_keep, R.end());
       const int t = add_tensor(tl);
@@ -156,14 +153,104 @@ struct Builder {
       return ew(t, -1, c);
     }
     op.nR = (int)R.size();
-    if (op.nR > 16) return fail(err, "more than 16 reduced labels in one EW step");
-    for (int i = 0; i < op.nR; i++) {
-      op.ra[i] = (int8_t)pos_in(LA, R[i]);
-      op.rb[i] = (int8_t)pos_in(LB, R[i]);
+    auto inA = [&](int32_t l) { return pos_in(LA, l) >= 0; };
+    auto inB = [&](int32_t l) { return pos_in(LB, l) >= 0; };
+    auto inR = [&](int32_t l) { return std::find(R.begin(), R.end(), l) != R.end(); };
+    int RA = 0, RB = 0;
+    for (int32_t l : R) RA += inA(l), RB += inB(l);
+    if (RA > kSubBits || RB > kSubBits) return fail(err, "EW step: reduced labels exceed a tile");
+    // tile labels: C's lowest positions, then enough of A's and B's lowest
+    // output labels that every operand is read contiguously, within the LDS
+    // budget; grown to >= 2^8 outputs where the output allows
+    std::vector<int32_t> T;
+    int nTA = 0, nTB = 0;
+    auto try_add = [&](int32_t l) {
+      if (std::find(T.begin(), T.end(), l) != T.end()) return true;
+      if ((int)T.size() >= kTileBits) return false;
+      const int ia = inA(l), ib = inB(l);
+      if (nTA + ia + RA > kSubBits || nTB + ib + RB > kSubBits) return false;
+      T.push_back(l);
+      nTA += ia;
+      nTB += ib;
+      return true;
+    };
+    auto cover = [&](const std::vector<int32_t>& L, int want) {
+      int got = 0;
+      for (int32_t l : L) {
+        if (got >= want) break;
+        if (inR(l) || try_add(l))
+          got++;
+        else
+          break;
+      }
+    };
+    // direct step: the tile is C's lowest min(nP, 10) positions and the
+    // operands are read straight from memory; taken when each operand is
+    // small (cache resident) or aligned: its lowest positions are lane labels
+    // (C positions 0..5) or reduced labels, so the loads are coalesced
+    {
+      const int nTd = std::min(op.nP, 10);
+      int na = 0, nb = 0;
+      for (int p = 0; p < nTd; p++) na += inA(LC[p]), nb += inB(LC[p]);
+      auto aligned = [&](const std::vector<int32_t>& L) {
+        for (int p = 0; p < std::min<int>(5, (int)L.size()); p++) {
+          const int pc = pos_in(LC, L[p]);
+          if (!inR(L[p]) && !(pc >= 0 && pc < std::min(6, nTd))) return false;
+        }
+        return true;
+      };
+      const bool okA = LA.size() <= 13 || aligned(LA);
+      const bool okB = b < 0 || LB.size() <= 13 || aligned(LB);
+      if (na + RA <= kSubBits && nb + RB <= kSubBits && okA && okB) {
+        T.assign(LC.begin(), LC.begin() + nTd);
+        op.direct = true;
+      }
     }
-    op.ept = std::max(0, std::min(4, op.nP - 6 - std::max(0, op.nR - 2)));
-    if (op.nP - 6 < op.ept) op.ept = std::max(0, op.nP - 6);
-    op.tiles = op.nP >= 6 ? (int64_t)1 << (op.nP - 6 - op.ept) : 1;
+    if (!op.direct) {
+      for (int p = 0; p < std::min(5, op.nP); p++) try_add(LC[p]);
+      cover(LA, 5);
+      cover(LB, 5);
+      // grow the tile while the budget allows: a tile's loads are its only
+      // latency hiding, so it should carry up to 2^12 outputs / sub-tensor
+      // entries
+      for (int p = 5; p < op.nP; p++) try_add(LC[p]);
+    }
+    std::sort(T.begin(), T.end(),
+              [&](int32_t x, int32_t y) { return pos_in(LC, x) < pos_in(LC, y); });
+    std::vector<int32_t> O;
+    for (int32_t l : LC)
+      if (std::find(T.begin(), T.end(), l) == T.end()) O.push_back(l);
+    std::vector<int32_t> SA, SB;  // sub-tensor labels in operand order
+    for (int32_t l : LA)
+      if (inR(l) || std::find(T.begin(), T.end(), l) != T.end()) SA.push_back(l);
+    for (int32_t l : LB)
+      if (inR(l) || std::find(T.begin(), T.end(), l) != T.end()) SB.push_back(l);
+    EwDesc& d = op.ew;
+    std::memset(&d, -1, sizeof d);
+    d.a = d.b = d.c = 0;
+    d.nT = (int)T.size();
+    d.nO = (int)O.size();
+    d.nR = op.nR;
+    d.nSA = (int)SA.size();
+    d.nSB = (int)SB.size();
+    d.tiles = 1 << d.nO;
+    for (int k = 0; k < d.nT; k++) {
+      d.tC[k] = (int8_t)pos_in(LC, T[k]);
+      d.tA[k] = (int8_t)pos_in(SA, T[k]);
+      d.tB[k] = (int8_t)pos_in(SB, T[k]);
+    }
+    for (int i = 0; i < op.nR; i++) {
+      d.rA[i] = (int8_t)pos_in(SA, R[i]);
+      d.rB[i] = (int8_t)pos_in(SB, R[i]);
+    }
+    for (int k = 0; k < d.nSA; k++) d.sA[k] = (int8_t)pos_in(LA, SA[k]);
+    for (int k = 0; k < d.nSB; k++) d.sB[k] = (int8_t)pos_in(LB, SB[k]);
+    for (int k = 0; k < d.nO; k++) {
+      d.oC[k] = (int8_t)pos_in(LC, O[k]);
+      d.oA[k] = (int8_t)pos_in(LA, O[k]);
+      d.oB[k] = (int8_t)pos_in(LB, O[k]);
+    }
+    op.tiles = d.tiles;
     op.macs = std::ldexp(1.0, op.nP + op.nR);
     op.bytes = 4.0 * (std::ldexp(1.0, (int)LA.size()) + (b >= 0 ? std::ldexp(1.0, (int)LB.size()) : 0) +
                       std::ldexp(1.0, op.nP));
@@ -193,6 +280,16 @@ struct Builder {
     return 0;
   }
 };
+
+// A node runs as a tiled GEMM when M, N, K are all large, or when K is large
+// enough that streaming it per output (2 loads per MAC) loses to register
+// blocking even in a partly filled 128x128 tile.
+bool gemm_shaped(const PlanOptions& opt, int na, int nb, int nk) {
+  const int64_t M = (int64_t)1 << std::min(na, 30), N = (int64_t)1 << std::min(nb, 30),
+                K = (int64_t)1 << std::min(nk, 30);
+  if (M >= 64 && N >= 64 && K >= 16) return true;
+  return M >= opt.gemm_min_m && N >= opt.gemm_min_n && K >= opt.gemm_min_k;
+}
 
 // Interval reuse of arena blocks: blocks freed by tensors that died at a level
 // strictly below a tensor's birth may be reused by it (exact size classes).
@@ -242,26 +339,53 @@ int verify_plan(const SlicePlan& P, std::string* err) {
     const int rb = op.b >= 0 ? P.tensors[op.b].rank : 0;
     const int rc = P.tensors[op.c].rank;
     if (op.kind == kEw) {
-      if (op.nP != rc || op.nR < 0 || op.nR > 16) return bad("EW ranks");
-      uint64_t ua = 0, ub = 0;
+      const EwDesc& d = op.ew;
+      if (op.nP != rc || d.nT + d.nO != rc || d.nT < 0 || d.nT > kTileBits || d.nO < 0 ||
+          d.nR != op.nR || d.nR < 0 || d.nR > kMaxReduce || d.nSA < 0 || d.nSA > kSubBits ||
+          d.nSB < 0 || d.nSB > kSubBits || (op.b < 0 && d.nSB != 0) ||
+          d.tiles != (1 << d.nO) || op.tiles != d.tiles || (op.direct && d.nT > 10))
+        return bad("EW shape");
+      // every position of A, B, C is covered exactly once, every sub-tensor
+      // bit by exactly one tile or reduced bit, and all indices in range
       auto take = [&](int8_t p, int r, uint64_t& used) {
         if (p < 0) return true;
         if (p >= r || ((used >> p) & 1)) return false;
         used |= 1ull << p;
         return true;
       };
-      for (int i = 0; i < op.nP; i++)
-        if (!take(op.pa[i], ra, ua) || !take(op.pb[i], rb, ub) || (op.b < 0 && op.pb[i] >= 0))
-          return bad("EW output positions");
-      for (int i = 0; i < op.nR; i++)
-        if (!take(op.ra[i], ra, ua) || !take(op.rb[i], rb, ub) || (op.b < 0 && op.rb[i] >= 0))
-          return bad("EW reduced positions");
-      if (ua != (ra >= 64 ? ~0ull : ((1ull << ra) - 1)) ||
-          (op.b >= 0 && ub != ((1ull << rb) - 1)))
-        return bad("EW step leaves an operand label unused");
-      const int tb = 6 + op.ept;
-      if (op.ept < 0 || op.ept > 4 || (op.ept > 0 && op.nP < tb)) return bad("EW ept");
-      if (op.tiles != (op.nP >= 6 ? (int64_t)1 << (op.nP - tb) : 1)) return bad("EW tiles");
+      uint64_t uc = 0, ua = 0, ub = 0, usa = 0, usb = 0;
+      for (int k = 0; k < d.nT; k++)
+        if (d.tC[k] < 0 || !take(d.tC[k], rc, uc) || !take(d.tA[k], d.nSA, usa) ||
+            !take(d.tB[k], d.nSB, usb))
+          return bad("EW tile bits");
+      for (int k = d.nT; k < kTileBits; k++)
+        if (d.tC[k] != -1 || d.tA[k] != -1 || d.tB[k] != -1) return bad("EW tile padding");
+      for (int i = 0; i < kMaxReduce; i++) {
+        if (i >= d.nR) {
+          if (d.rA[i] != -1 || d.rB[i] != -1) return bad("EW reduced padding");
+          continue;
+        }
+        if (!take(d.rA[i], d.nSA, usa) || !take(d.rB[i], d.nSB, usb)) return bad("EW reduced bits");
+      }
+      for (int k = 0; k < kSubBits; k++) {
+        if (k >= d.nSA && d.sA[k] != -1) return bad("EW sub-tensor padding");
+        if (k >= d.nSB && d.sB[k] != -1) return bad("EW sub-tensor padding");
+        if (k < d.nSA && !take(d.sA[k], ra, ua)) return bad("EW sub-tensor A");
+        if (k < d.nSB && !take(d.sB[k], rb, ub)) return bad("EW sub-tensor B");
+      }
+      for (int k = 0; k < 32; k++) {
+        if (k >= d.nO) {
+          if (d.oC[k] != -1 || d.oA[k] != -1 || d.oB[k] != -1) return bad("EW outer padding");
+          continue;
+        }
+        if (d.oC[k] < 0 || !take(d.oC[k], rc, uc) || !take(d.oA[k], ra, ua) ||
+            !take(d.oB[k], rb, ub))
+          return bad("EW outer bits");
+      }
+      auto full = [](int r) { return r >= 64 ? ~0ull : ((1ull << r) - 1); };
+      if (uc != full(rc) || ua != full(ra) || (op.b >= 0 && ub != full(rb)) ||
+          usa != full(d.nSA) || usb != full(d.nSB))
+        return bad("EW step leaves a position uncovered");
     } else {
       const int64_t ea = (int64_t)op.M * op.K * op.batch, eb = (int64_t)op.K * op.N * op.batch,
                     ec = (int64_t)op.M * op.N * op.batch;
@@ -459,8 +583,7 @@ int plan_slice(const SliceIn& s, const PlanOptions& opt, SlicePlan* out, std::st
       Cls& c = cls[j];
       const int o = nl + j;
       int na = (int)c.a.size(), nb = (int)c.b.size(), nk = (int)c.k.size();
-      bool g = !c.priv && (1 << std::min(na, 30)) >= opt.gemm_min_m &&
-               (1 << std::min(nb, 30)) >= opt.gemm_min_n && (1 << std::min(nk, 30)) >= opt.gemm_min_k;
+      bool g = !c.priv && gemm_shaped(opt, na, nb, nk);
       is_gemm[j] = g;
       if (g) {
         auto sa = sorted_key(c.a), sb = sorted_key(c.b), sk = sorted_key(c.k), st = sorted_key(c.t);
@@ -531,8 +654,7 @@ int plan_slice(const SliceIn& s, const PlanOptions& opt, SlicePlan* out, std::st
       int dst = o;
       if (gout != LC) dst = B.add_tensor(gout);
       int na = (int)c.a.size(), nb = (int)c.b.size(), nk = (int)c.k.size();
-      bool g = (1 << std::min(na, 30)) >= opt.gemm_min_m && (1 << std::min(nb, 30)) >= opt.gemm_min_n &&
-               (1 << std::min(nk, 30)) >= opt.gemm_min_k;
+      bool g = gemm_shaped(opt, na, nb, nk);
       if (g)
         B.gemm(ta, tb, dst, na, nb, nk, (int)c.t.size());
       else if (B.ew(ta, tb, dst))

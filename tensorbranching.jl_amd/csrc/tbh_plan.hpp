@@ -36,10 +36,11 @@ struct POp {
   int kind = kEw;
   int a = -1, b = -1, c = -1;  // tensor ids (b = -1: unary)
   int level = 0;               // wave (batched) or sequence index (plain)
-  // EW
-  int nP = 0, nR = 0, ept = 0;
+  // EW: the step's descriptor with device addresses left 0 (set at launch)
+  int nP = 0, nR = 0;
   int64_t tiles = 0;
-  int8_t pa[32], pb[32], ra[16], rb[16];
+  bool direct = false;  // wave tiles of <= 2^10 outputs read straight from memory
+  EwDesc ew{};
   // GEMM
   int32_t M = 0, N = 0, K = 0, batch = 0;
   double macs = 0;     // executed (output x reduced) configurations
@@ -67,7 +68,9 @@ struct SlicePlan {
 
 struct PlanOptions {
   bool batched = true;   // wave levels + layout planning; false: OMEinsum lowering
-  int gemm_min_m = 64, gemm_min_n = 64, gemm_min_k = 16;
+  // GEMM-shaped nodes: M, N, K (and the product of the reduction with the
+  // outputs) large enough that register blocking beats streaming
+  int gemm_min_m = 16, gemm_min_n = 16, gemm_min_k = 64;
 };
 
 // Returns 0 or sets *err and returns -1.

@@ -101,6 +101,53 @@ Staging::~Staging() {
   if (host) (void)hipHostFree(host);
 }
 
+// ------------------------------------------------------- profiler --------
+hipEvent_t Profiler::get() {
+  if (used == pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    pool.push_back(e);
+  }
+  return pool[used++];
+}
+
+hipEvent_t Profiler::begin(hipStream_t s) {
+  hipEvent_t a = get();
+  if (a) (void)hipEventRecord(a, s);
+  return a;
+}
+
+void Profiler::end(hipStream_t s, hipEvent_t a, int kind, int level, int nops, double bytes,
+                   double macs, const char* tag) {
+  hipEvent_t b = get();
+  if (!a || !b) return;
+  (void)hipEventRecord(b, s);
+  recs.push_back({a, b, kind, level, nops, bytes, macs, tag ? tag : ""});
+}
+
+void Profiler::collect(tbh_stats& st) {
+  const char* path = std::getenv("TBH_PROFILE_LEVELS");
+  FILE* f = path ? std::fopen(path, "a") : nullptr;
+  for (const Rec& r : recs) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
+    st.kind_ms[r.kind] += ms;
+    st.kind_bytes[r.kind] += r.bytes;
+    st.kind_macs[r.kind] += r.macs;
+    st.kind_launches[r.kind]++;
+    if (f)
+      std::fprintf(f, "%d %d %d %.6f %.6g %.6g %s\n", r.level, r.kind, r.nops, ms, r.bytes, r.macs,
+                   r.tag.c_str());
+  }
+  if (f) std::fclose(f);
+  recs.clear();
+  used = 0;
+}
+
+Profiler::~Profiler() {
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
 // ------------------------------------------------------- conversion ------
 static uint32_t f2b(float f) {
   uint32_t u;
@@ -226,19 +273,10 @@ struct Pack {
 };
 
 EwDesc make_ew(const POp& op, const std::vector<PTensor>& T, uint64_t base) {
-  EwDesc d;
-  std::memset(&d, 0, sizeof d);
+  EwDesc d = op.ew;
   d.a = base + (uint64_t)T[op.a].off;
   d.b = op.b >= 0 ? base + (uint64_t)T[op.b].off : 0;
   d.c = base + (uint64_t)T[op.c].off;
-  d.nP = op.nP;
-  d.nR = op.nR;
-  d.ept = op.ept;
-  d.tiles = (int32_t)op.tiles;
-  std::memcpy(d.pa, op.pa, 32);
-  std::memcpy(d.pb, op.pb, 32);
-  std::memcpy(d.ra, op.ra, 16);
-  std::memcpy(d.rb, op.rb, 16);
   return d;
 }
 
@@ -265,17 +303,24 @@ GemmDesc make_gemm(const POp& op, const std::vector<PTensor>& T, uint64_t base) 
 
 int64_t gemm_tiles(const GemmDesc& d) { return (int64_t)d.tiles_m * d.tiles_n * d.batch; }
 
+// EW workgroups: one tile chunk each, at most 4 resident per CU (LDS 33 KB)
 int ew_grid(tbh_ctx* ctx, int64_t total, int chunk) {
-  int64_t waves = (total + chunk - 1) / chunk;
-  int64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)ctx->num_cus * 8));
+  int64_t blocks = (total + chunk - 1) / chunk;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)ctx->num_cus * 4));
 }
 
 int gemm_grid(tbh_ctx* ctx, int64_t total) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)ctx->num_cus * 2));
 }
 
-constexpr int kChunk = 4;
+// direct steps: wave tiles, 4 waves per workgroup
+int direct_grid(tbh_ctx* ctx, int64_t total, int chunk) {
+  const int64_t waves = (total + chunk - 1) / chunk;
+  return (int)std::max<int64_t>(1, std::min<int64_t>((waves + 3) / 4, (int64_t)ctx->num_cus * 8));
+}
+
+constexpr int kChunk = 2;
+constexpr int kChunkDirect = 4;
 
 }  // namespace
 
@@ -320,8 +365,17 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     int H = 0;
     for (size_t i = i0; i < i1; i++) H = std::max(H, plans[i]->nlevels);
     // gather steps by level
-    std::vector<std::vector<EwDesc>> ew(H + 1);
+    std::vector<std::vector<EwDesc>> ew(H + 1), ews(H + 1);
     std::vector<std::vector<GemmDesc>> gm(H + 1);
+    std::vector<double> lvl_bytes[3], lvl_macs[3];  // per kernel kind and level
+    std::vector<std::vector<double>> ew_bytes(H + 1), ew_macs(H + 1);  // tiled steps
+    std::vector<std::vector<std::pair<int, int>>> ew_ranks(H + 1);
+    std::vector<std::vector<double>> ews_bytes(H + 1), ews_macs(H + 1);  // direct steps
+    std::vector<std::vector<std::pair<int, int>>> ews_ranks(H + 1);
+    for (int k = 0; k < 3; k++) {
+      lvl_bytes[k].assign(H + 1, 0.0);
+      lvl_macs[k].assign(H + 1, 0.0);
+    }
     std::vector<LeafDesc> leaves;
     std::vector<EpiDesc> epi(i1 - i0);
     for (size_t i = i0; i < i1; i++) {
@@ -344,11 +398,20 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
         leaves.push_back(d);
       }
       for (const POp& op : P.ops) {
+        const int kind = op.kind == kGemm ? 2 : (op.direct ? 0 : 1);
         if (op.kind == kEw) {
-          ew[op.level].push_back(make_ew(op, P.tensors, b));
+          (op.direct ? ews : ew)[op.level].push_back(make_ew(op, P.tensors, b));
+          auto& xb = op.direct ? ews_bytes : ew_bytes;
+          auto& xm = op.direct ? ews_macs : ew_macs;
+          auto& xr = op.direct ? ews_ranks : ew_ranks;
+          xb[op.level].push_back(op.bytes);
+          xm[op.level].push_back(op.macs);
+          xr[op.level].push_back({P.tensors[op.a].rank, op.b >= 0 ? P.tensors[op.b].rank : -1});
         } else {
           gm[op.level].push_back(make_gemm(op, P.tensors, b));
         }
+        lvl_bytes[kind][op.level] += op.bytes;
+        lvl_macs[kind][op.level] += op.macs;
         ctx->stats.ew_ops += op.kind == kEw;
         ctx->stats.gemm_ops += op.kind == kGemm;
         if (op.kind == kEw) ctx->stats.ew_ops_algo_bytes += op.bytes;
@@ -363,16 +426,35 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     const size_t o_epi = pk.put(epi.data(), epi.size() * sizeof(EpiDesc));
     const size_t o_res = pk.reserve(4 * (i1 - i0));
     struct LvlOff {
-      size_t ew, ewp, gm, gmp;
-      int64_t ew_total, gm_total;
+      size_t ew, ewp, ews, ewsp, gm, gmp, one, onew;
+      int64_t ew_total, ews_total, gm_total;
     };
+    const bool per_step = ctx->profiling && std::getenv("TBH_PROFILE_STEPS");
     std::vector<LvlOff> lo(H + 1);
     for (int L = 1; L <= H; L++) {
+      if (per_step) {
+        std::vector<int64_t> one, onew;
+        for (const EwDesc& d : ew[L]) {
+          one.push_back(0);
+          one.push_back(d.tiles);
+        }
+        for (const EwDesc& d : ews[L]) {
+          onew.push_back(0);
+          onew.push_back(d.tiles);
+        }
+        lo[L].one = pk.put(one.data(), one.size() * sizeof(int64_t));
+        lo[L].onew = pk.put(onew.data(), onew.size() * sizeof(int64_t));
+      }
       std::vector<int64_t> pe(ew[L].size() + 1, 0), pg(gm[L].size() + 1, 0);
       for (size_t k = 0; k < ew[L].size(); k++) pe[k + 1] = pe[k] + ew[L][k].tiles;
       for (size_t k = 0; k < gm[L].size(); k++) pg[k + 1] = pg[k] + gemm_tiles(gm[L][k]);
       lo[L].ew = pk.put(ew[L].data(), ew[L].size() * sizeof(EwDesc));
       lo[L].ewp = pk.put(pe.data(), pe.size() * sizeof(int64_t));
+      lo[L].ews = pk.put(ews[L].data(), ews[L].size() * sizeof(EwDesc));
+      std::vector<int64_t> pd(ews[L].size() + 1, 0);
+      for (size_t k = 0; k < ews[L].size(); k++) pd[k + 1] = pd[k] + ews[L][k].tiles;
+      lo[L].ewsp = pk.put(pd.data(), pd.size() * sizeof(int64_t));
+      lo[L].ews_total = pd.back();
       lo[L].gm = pk.put(gm[L].data(), gm[L].size() * sizeof(GemmDesc));
       lo[L].gmp = pk.put(pg.data(), pg.size() * sizeof(int64_t));
       lo[L].ew_total = pe.back();
@@ -413,17 +495,62 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     hipError_t e = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), (int64_t)leaves.size(),
                                  ctx->stream);
     ctx->stats.launches += leaves.empty() ? 0 : 1;
+    const bool prof = ctx->profiling;
+    hipEvent_t ev = nullptr;
     for (int L = 1; L <= H && e == hipSuccess; L++) {
-      if (lo[L].ew_total > 0) {
+      if (!ews[L].empty() && per_step) {
+        for (size_t k = 0; k < ews[L].size() && e == hipSuccess; k++) {
+          const EwDesc& d = ews[L][k];
+          if (d.nO < 8) continue;  // only the big ones
+          ev = ctx->prof.begin(ctx->stream);
+          e = launch_ew_direct(dtype, (const EwDesc*)(dbase + lo[L].ews) + k,
+                               (const int64_t*)(dbase + lo[L].onew) + 2 * k, 1, d.tiles,
+                               direct_grid(ctx, d.tiles, kChunkDirect), kChunkDirect, ctx->stream);
+          char tag[128];
+          std::snprintf(tag, sizeof tag, "nT=%d nO=%d nR=%d nSA=%d nSB=%d ra=%d rb=%d", d.nT,
+                        d.nO, d.nR, d.nSA, d.nSB, ews_ranks[L][k].first, ews_ranks[L][k].second);
+          ctx->prof.end(ctx->stream, ev, 0, L, 1, ews_bytes[L][k], ews_macs[L][k], tag);
+        }
+      } else if (!ews[L].empty()) {
+        if (prof) ev = ctx->prof.begin(ctx->stream);
+        e = launch_ew_direct(dtype, (const EwDesc*)(dbase + lo[L].ews),
+                             (const int64_t*)(dbase + lo[L].ewsp), (int)ews[L].size(),
+                             lo[L].ews_total, direct_grid(ctx, lo[L].ews_total, kChunkDirect),
+                             kChunkDirect, ctx->stream);
+        if (prof)
+          ctx->prof.end(ctx->stream, ev, 0, L, (int)ews[L].size(), lvl_bytes[0][L], lvl_macs[0][L]);
+        ctx->stats.launches++;
+      }
+      if (e == hipSuccess && lo[L].ew_total > 0 && per_step) {
+        // profiling of single steps: each tiled step alone, logged with its shape
+        for (size_t k = 0; k < ew[L].size() && e == hipSuccess; k++) {
+          const EwDesc& d = ew[L][k];
+          ev = ctx->prof.begin(ctx->stream);
+          e = launch_ew(dtype, (const EwDesc*)(dbase + lo[L].ew) + k,
+                        (const int64_t*)(dbase + lo[L].one) + 2 * k, 1, d.tiles,
+                        ew_grid(ctx, d.tiles, kChunk), kChunk, ctx->stream);
+          char tag[128];
+          std::snprintf(tag, sizeof tag, "nT=%d nO=%d nR=%d nSA=%d nSB=%d ra=%d rb=%d", d.nT,
+                        d.nO, d.nR, d.nSA, d.nSB, ew_ranks[L][k].first, ew_ranks[L][k].second);
+          ctx->prof.end(ctx->stream, ev, 1, L, 1, ew_bytes[L][k], ew_macs[L][k], tag);
+          ctx->stats.launches++;
+        }
+      } else if (e == hipSuccess && lo[L].ew_total > 0) {
+        if (prof) ev = ctx->prof.begin(ctx->stream);
         e = launch_ew(dtype, (const EwDesc*)(dbase + lo[L].ew), (const int64_t*)(dbase + lo[L].ewp),
                       (int)ew[L].size(), lo[L].ew_total, ew_grid(ctx, lo[L].ew_total, kChunk),
                       kChunk, ctx->stream);
+        if (prof)
+          ctx->prof.end(ctx->stream, ev, 1, L, (int)ew[L].size(), lvl_bytes[1][L], lvl_macs[1][L]);
         ctx->stats.launches++;
       }
       if (e == hipSuccess && lo[L].gm_total > 0) {
+        if (prof) ev = ctx->prof.begin(ctx->stream);
         e = launch_gemm(dtype, true, (const GemmDesc*)(dbase + lo[L].gm),
                         (const int64_t*)(dbase + lo[L].gmp), (int)gm[L].size(), lo[L].gm_total,
                         gemm_grid(ctx, lo[L].gm_total), ctx->stream);
+        if (prof)
+          ctx->prof.end(ctx->stream, ev, 2, L, (int)gm[L].size(), lvl_bytes[2][L], lvl_macs[2][L]);
         ctx->stats.launches++;
       }
       ctx->stats.waves++;
@@ -437,6 +564,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
       e = hipMemcpyAsync(ctx->staging.host, dbase + o_res, 4 * (i1 - i0), hipMemcpyDeviceToHost,
                          ctx->stream);
     hipError_t es = hipStreamSynchronize(ctx->stream);
+    if (prof) ctx->prof.collect(ctx->stats);
     if (check(e, "batched launch") || check(es, "batched sync")) {
       cleanup();
       return TBH_EDEVICE;
@@ -518,9 +646,16 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
   ctx->stats.launches++;
   for (size_t k = 0; k < P.ops.size() && err == hipSuccess; k++) {
     const POp& op = P.ops[k];
+    hipEvent_t ev = ctx->profiling ? ctx->prof.begin(ctx->stream) : nullptr;
     if (op.kind == kEw) {
-      err = launch_ew(dtype, (const EwDesc*)(dbase + o_op[k]), (const int64_t*)(dbase + o_pre[k]),
-                      1, tiles[k], ew_grid(ctx, tiles[k], kChunk), kChunk, ctx->stream);
+      if (op.direct)
+        err = launch_ew_direct(dtype, (const EwDesc*)(dbase + o_op[k]),
+                               (const int64_t*)(dbase + o_pre[k]), 1, tiles[k],
+                               direct_grid(ctx, tiles[k], kChunkDirect), kChunkDirect, ctx->stream);
+      else
+        err = launch_ew(dtype, (const EwDesc*)(dbase + o_op[k]),
+                        (const int64_t*)(dbase + o_pre[k]), 1, tiles[k],
+                        ew_grid(ctx, tiles[k], kChunk), kChunk, ctx->stream);
       ctx->stats.ew_ops++;
       ctx->stats.ew_ops_algo_bytes += op.bytes;
     } else {
@@ -530,6 +665,9 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
       ctx->stats.gemm_ops++;
       ctx->stats.gemm_ops_algo_ops += 2 * op.macs;
     }
+    if (ev)
+      ctx->prof.end(ctx->stream, ev, op.kind == kGemm ? 2 : (op.direct ? 0 : 1), (int)k + 1, 1,
+                    op.bytes, op.macs);
     ctx->stats.launches++;
   }
   ctx->stats.total_algo_ops += 2 * P.node_macs;
@@ -540,6 +678,7 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
   if (err == hipSuccess)
     err = hipMemcpyAsync(ctx->staging.host, dbase + o_res, 4, hipMemcpyDeviceToHost, ctx->stream);
   hipError_t es = hipStreamSynchronize(ctx->stream);
+  if (ctx->profiling) ctx->prof.collect(ctx->stats);
   ctx->pool.free(doff);
   ctx->pool.free(arena_off);
   if (err != hipSuccess || es != hipSuccess) {
@@ -783,13 +922,18 @@ int tbh_maxplus_gemm(tbh_ctx* ctx, tbh_dtype dtype, int64_t M, int64_t N, int64_
   std::memcpy((char*)ctx->staging.host + sizeof d, pre, sizeof pre);
   hipError_t e = hipMemcpyAsync(dd, ctx->staging.host, sizeof d + sizeof pre,
                                 hipMemcpyHostToDevice, ctx->stream);
+  hipEvent_t ev = ctx->profiling ? ctx->prof.begin(ctx->stream) : nullptr;
   if (e == hipSuccess)
     e = launch_gemm(dtype, vec, (const GemmDesc*)dd, (const int64_t*)(dd + sizeof d), 1, total,
                     gemm_grid(ctx, total), ctx->stream);
+  if (ev)
+    ctx->prof.end(ctx->stream, ev, 2, 0, 1, 4.0 * (double)batch * (M * K + K * N + M * N),
+                  (double)M * N * K * batch);
   ctx->stats.launches++;
   if (e == hipSuccess && ptr_kind == TBH_HOST)
     e = hipMemcpyAsync(C, dC, 4 * ec, hipMemcpyDeviceToHost, ctx->stream);
   hipError_t es = hipStreamSynchronize(ctx->stream);
+  if (ctx->profiling) ctx->prof.collect(ctx->stats);
   release();
   if (e != hipSuccess || es != hipSuccess) {
     set_error(std::string("maxplus_gemm: ") + hipGetErrorString(e != hipSuccess ? e : es));
@@ -866,6 +1010,49 @@ int tbh_slice_footprint(const tbh_slice* slice, tbh_dtype dtype, size_t* bytes) 
   return 0;
 }
 
+int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode, tbh_step_info* out,
+                   int cap) {
+  SliceOwned o;
+  if (!slice || (cap > 0 && !out)) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  int rc = convert_slice(*slice, dtype, false, &o);
+  if (rc) return rc;
+  SlicePlan P;
+  PlanOptions opt;
+  opt.batched = mode == TBH_BATCHED;
+  std::string err;
+  if (plan_slice(o.in, opt, &P, &err)) {
+    set_error(err);
+    return TBH_EINVAL;
+  }
+  int n = (int)P.ops.size();
+  for (int i = 0; i < n && i < cap; i++) {
+    const POp& op = P.ops[i];
+    tbh_step_info& s = out[i];
+    std::memset(&s, 0, sizeof s);
+    s.kind = op.kind;
+    s.level = op.level;
+    s.rank_a = P.tensors[op.a].rank;
+    s.rank_b = op.b >= 0 ? P.tensors[op.b].rank : -1;
+    s.rank_c = P.tensors[op.c].rank;
+    s.n_reduced = op.nR;
+    s.M = op.M;
+    s.N = op.N;
+    s.K = op.K;
+    s.batch = op.batch;
+    // how many of A's / B's lowest positions a tile reads contiguously
+    if (op.kind == kEw) {
+      for (int k = 0; k < op.ew.nSA && op.ew.sA[k] == k; k++) s.lanes_a++;
+      for (int k = 0; k < op.ew.nSB && op.ew.sB[k] == k; k++) s.lanes_b++;
+    }
+    s.macs = op.macs;
+    s.bytes = op.bytes;
+  }
+  return n;
+}
+
 int tbh_get_stats(tbh_ctx* ctx, tbh_stats* out) {
   if (!ctx || !out) {
     set_error("null argument");
@@ -875,6 +1062,16 @@ int tbh_get_stats(tbh_ctx* ctx, tbh_stats* out) {
   *out = ctx->stats;
   out->pool_bytes = ctx->pool.capacity();
   out->pool_peak_bytes = ctx->pool.peak();
+  return 0;
+}
+
+int tbh_set_profiling(tbh_ctx* ctx, int on) {
+  if (!ctx) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->profiling = on != 0;
   return 0;
 }
 

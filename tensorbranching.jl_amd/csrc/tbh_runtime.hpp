@@ -24,6 +24,8 @@ void set_error(const std::string& m);
 // launchers (tbh_kernels.hip)
 hipError_t launch_ew(int dtype, const EwDesc* ops, const int64_t* prefix, int nops, int64_t total,
                      int grid, int chunk, hipStream_t s);
+hipError_t launch_ew_direct(int dtype, const EwDesc* ops, const int64_t* prefix, int nops,
+                            int64_t total, int grid, int chunk, hipStream_t s);
 hipError_t launch_gemm(int dtype, bool vec, const GemmDesc* ops, const int64_t* prefix, int nops,
                        int64_t total, int grid, hipStream_t s);
 hipError_t launch_leaves(int dtype, const LeafDesc* leaves, int64_t n, hipStream_t s);
@@ -56,6 +58,27 @@ struct Staging {  // pinned host buffer reused for descriptor uploads
 
 struct ServeState;
 
+// HIP-event timing of launches on the context's stream (profiling mode).
+struct Profiler {
+  struct Rec {
+    hipEvent_t a, b;
+    int kind, level, nops;
+    double bytes, macs;
+    std::string tag;
+  };
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<Rec> recs;
+  hipEvent_t get();
+  // record around one launch: call begin() before and end() after it
+  hipEvent_t begin(hipStream_t s);
+  void end(hipStream_t s, hipEvent_t a, int kind, int level, int nops, double bytes, double macs,
+           const char* tag = nullptr);
+  // after the stream is synchronized: fold the records into stats
+  void collect(tbh_stats& st);
+  ~Profiler();
+};
+
 }  // namespace tbh
 
 struct tbh_ctx {
@@ -68,6 +91,8 @@ struct tbh_ctx {
   std::mutex mu;         // one executor call at a time
   tbh_stats stats{};
   int plan_threads = 8;
+  bool profiling = false;
+  tbh::Profiler prof;
   std::shared_ptr<tbh::ServeState> serve;  // created by tbh_serve_start
 };
 

@@ -1,0 +1,130 @@
+# TensorBranchingHIP.jl -- ccall shim binding libtbh.so (include/tbh.h) into
+# TensorBranching.jl's contraction step.
+#
+# Drop-in for contract_slices / solve_slice (src/dynamic_ob.jl:30-48): the
+# same names and arguments, with the backend object in place of usecuda.
+# Slices are marshalled straight from SlicedBranch / CompressedEinsum
+# (src/types.jl:51-58, 85-103) without uncompress; labels and leaf indices
+# become 0-based.  Errors come back as Julia exceptions carrying
+# tbh_last_error().
+module TensorBranchingHIP
+
+using TensorBranching
+using TensorBranching: SlicedBranch, CompressedEinsum
+using TensorBranching.OMEinsum.OMEinsumContractionOrders: ContractionTree
+using TensorBranching.Graphs: nv
+using TensorBranching.OptimalBranching.OptimalBranchingMIS: UnitWeight
+
+const libtbh = get(ENV, "TBH_LIB", joinpath(@__DIR__, "..", "lib", "libtbh.so"))
+
+const TBH_F32 = Cint(0)
+const TBH_I32 = Cint(1)
+const TBH_PLAIN = Cint(0)
+const TBH_BATCHED = Cint(1)
+
+struct TbhSlice            # tbh_slice
+    nv::Int32
+    weights::Ptr{Cvoid}
+    nleaves::Int32
+    ix_ptr::Ptr{Int32}
+    ix::Ptr{Int32}
+    tree::Ptr{Int32}
+    r::Float64
+end
+
+mutable struct HIPBackend
+    ctx::Ptr{Cvoid}
+    mode::Cint
+    function HIPBackend(; device::Integer = 0, pool_bytes::Integer = 0, batched::Bool = true)
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:tbh_create, libtbh), Cint, (Cint, Csize_t, Ptr{Ptr{Cvoid}}),
+                   device, pool_bytes, out)
+        rc == 0 || error(unsafe_string(ccall((:tbh_last_error, libtbh), Cstring, ())))
+        b = new(out[], batched ? TBH_BATCHED : TBH_PLAIN)
+        finalizer(b) do x
+            x.ctx == C_NULL || ccall((:tbh_destroy, libtbh), Cint, (Ptr{Cvoid},), x.ctx)
+            x.ctx = C_NULL
+        end
+        return b
+    end
+end
+
+dtype_code(::Type{Float32}) = TBH_F32
+dtype_code(::Type{Int32}) = TBH_I32
+dtype_code(T::Type) = error("element_type must be Float32 or Int32, got $T")
+
+# post-order child list of a ContractionTree whose leaves index ixs (1-based)
+function postorder(ct, nleaves::Int)
+    out = Int32[]
+    function walk(node)
+        node isa Integer && return Int32(node - 1)
+        l = walk(node.left)
+        r = walk(node.right)
+        push!(out, l, r)
+        return Int32(nleaves + length(out) ÷ 2 - 1)
+    end
+    walk(ct)
+    return out
+end
+
+# flat buffers of one branch; kept alive by the caller around the ccall
+function marshal(b::SlicedBranch, ::Type{T}) where {T}
+    g = b.p.g
+    w = b.p.weights isa UnitWeight ? nothing : T.(b.p.weights)
+    if nv(g) == 0 || b.code === nothing
+        return (w, Int32[], Int32[], Int32[], 0)
+    end
+    ixs = b.code.ixs
+    ix_ptr = Int32[0; cumsum(Int32.(length.(ixs)))]
+    ix = Int32[l - 1 for x in ixs for l in x]
+    tree = postorder(b.code.ct, length(ixs))
+    return (w, ix_ptr, ix, tree, length(ixs))
+end
+
+function contract_slices(branches::Vector{<:SlicedBranch}, element_type::Type, be::HIPBackend)
+    bufs = [marshal(b, element_type) for b in branches]
+    res = Vector{element_type}(undef, length(branches))
+    GC.@preserve bufs res begin
+        slices = [TbhSlice(nv(b.p.g),
+                           buf[1] === nothing ? C_NULL : pointer(buf[1]),
+                           Int32(buf[5]), pointer(buf[2]), pointer(buf[3]), pointer(buf[4]),
+                           Float64(b.r)) for (b, buf) in zip(branches, bufs)]
+        GC.@preserve slices begin
+            rc = ccall((:tbh_contract_slices, libtbh), Cint,
+                       (Ptr{Cvoid}, Cint, Ptr{TbhSlice}, Csize_t, Ptr{Cvoid}, Cint),
+                       be.ctx, dtype_code(element_type), slices, length(slices), res, be.mode)
+        end
+    end
+    rc == 0 || error(unsafe_string(ccall((:tbh_last_error, libtbh), Cstring, ())))
+    return res
+end
+
+function solve_slice(b::SlicedBranch, element_type::Type, be::HIPBackend)
+    buf = marshal(b, element_type)
+    res = Ref{element_type}()
+    GC.@preserve buf begin
+        s = Ref(TbhSlice(nv(b.p.g), buf[1] === nothing ? C_NULL : pointer(buf[1]),
+                         Int32(buf[5]), pointer(buf[2]), pointer(buf[3]), pointer(buf[4]),
+                         Float64(b.r)))
+        rc = ccall((:tbh_solve_slice, libtbh), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{TbhSlice}, Ptr{Cvoid}, Cint),
+                   be.ctx, dtype_code(element_type), s, res, be.mode)
+    end
+    rc == 0 || error(unsafe_string(ccall((:tbh_last_error, libtbh), Cstring, ())))
+    return res[]
+end
+
+# dynamic_ob_mis with the contraction step on the MI355X (src/dynamic_ob.jl:6-28)
+function dynamic_ob_mis(g, weights, be::HIPBackend; reducer = TensorBranching.TensorNetworkReducer(),
+                        slicer = TensorBranching.ContractionTreeSlicer(),
+                        optimizer = TensorBranching.TreeSA(), element_type::Type = Float32,
+                        verbose::Int = 0)
+    res = TensorBranching.kernelize(g, weights, reducer, verbose = verbose)
+    nv(res.g) == 0 && return res.r
+    code = TensorBranching.initialize_code(res.g, optimizer)
+    branches = TensorBranching.slice(TensorBranching.MISProblem(res.g, res.weights), code, res.r,
+                                     slicer, reducer, verbose = verbose)
+    return maximum(contract_slices(branches, element_type, be))
+end
+
+end # module

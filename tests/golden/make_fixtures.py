@@ -1,0 +1,91 @@
+"""Writes the committed fixtures under tests/golden/.
+
+The reference has no stored vectors (its tests check invariants only), so
+these are produced by this package's generators and slicer and checked,
+before they are written, against solvers independent of the contraction:
+the exact branch-and-reduce MIS, brute force (small graphs) and the forest
+DP.  Run from the repository root:  python tests/golden/make_fixtures.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+
+def dump_branches(br):
+    out = []
+    for b in br:
+        item = {"nv": b.p.g.nv, "edges": b.p.g.edges.reshape(-1).tolist(), "r": float(b.r),
+                "weights": None if b.p.weights is None else np.asarray(b.p.weights).tolist()}
+        if b.code is not None:
+            item.update(ix_ptr=b.code.ix_ptr.tolist(), ix=b.code.ix.tolist(),
+                        tree=b.code.tree.tolist())
+        out.append(item)
+    return out
+
+
+def main():
+    import __graft_entry__
+    from oracle import oracle
+    tb = __graft_entry__.package()
+
+    # config 1: 3-regular n=100, seed 0, sliced to sc_target 10, unit weights
+    g = tb.random_regular_graph(100, 3, seed=0)
+    br = tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=10, order_trials=20), seed=0)
+    res = oracle.contract_slices(br, np.float32)
+    mis = oracle.mis_exact(g)
+    assert res.max() == mis
+    json.dump({"graph": {"nv": g.nv, "edges": g.edges.reshape(-1).tolist()},
+               "dtype": "float32", "slices": dump_branches(br),
+               "per_slice": res.tolist(), "mis": mis},
+              open(os.path.join(HERE, "c1_regular100.json"), "w"))
+
+    # weighted Int32: 6x6 King's subgraph, filling 0.8, weights in [1, 100]
+    g = tb.random_ksg(6, 6, 0.8, seed=3)
+    w = np.random.default_rng(3).integers(1, 101, g.nv).astype(np.int32)
+    br = tb.slice_graph(g, w, tb.ContractionTreeSlicer(sc_target=5, order_trials=10), seed=3)
+    res = oracle.contract_slices(br, np.int32)
+    mwis = oracle.mis_bruteforce(g, w.astype(np.float64))
+    assert int(res.max()) == int(mwis)
+    json.dump({"graph": {"nv": g.nv, "edges": g.edges.reshape(-1).tolist()},
+               "weights": w.tolist(), "dtype": "int32", "slices": dump_branches(br),
+               "per_slice": res.tolist(), "mis": int(mwis)},
+              open(os.path.join(HERE, "ksg6_int32.json"), "w"))
+
+    # a weighted tree, pinned by the forest DP
+    rng = np.random.default_rng(5)
+    n = 30
+    g = tb.SimpleGraph(n, [[int(rng.integers(0, i)), i] for i in range(1, n)])
+    w = rng.integers(1, 50, n).astype(np.float32)
+    code = tb.initialize_code(g, 4, 5)
+    br = [tb.SlicedBranch(tb.MISProblem(g, w), code, 0)]
+    res = oracle.contract_slices(br, np.float32)
+    assert res[0] == oracle.mwis_forest(g, w.astype(np.float64))
+    json.dump({"graph": {"nv": g.nv, "edges": g.edges.reshape(-1).tolist()},
+               "weights": w.tolist(), "dtype": "float32", "slices": dump_branches(br),
+               "per_slice": res.tolist(), "mis": float(res[0])},
+              open(os.path.join(HERE, "tree30_f32.json"), "w"))
+
+    # max-plus GEMM: 5x7 by 7x3, integers with -inf, C checked by enumeration
+    A = rng.integers(0, 20, (5, 7)).astype(np.float32)
+    B = rng.integers(0, 20, (7, 3)).astype(np.float32)
+    A[rng.random(A.shape) < 0.2] = -np.inf
+    C = np.full((5, 3), -np.inf, np.float32)
+    for i in range(5):
+        for j in range(3):
+            for k in range(7):
+                C[i, j] = max(C[i, j], A[i, k] + B[k, j])
+    assert np.array_equal(C, oracle.maxplus_gemm(np.asfortranarray(A), np.asfortranarray(B)))
+    enc = lambda M: [[None if np.isneginf(x) else float(x) for x in row] for row in M]
+    json.dump({"A": enc(A), "B": enc(B), "C": enc(C)},
+              open(os.path.join(HERE, "gemm_5x7x3.json"), "w"))
+    print("fixtures written")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,102 @@
+"""The C-ABI libraries load without a GPU and export every function their
+headers declare; host-only entry points behave (no device calls here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header, prefix):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(" + prefix + r"_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_tbh_exports_everything_declared(tb):
+    from tensorbranching_jl_amd import _lib
+    lib = _lib.tbh()
+    names = _declared("tbh.h", "tbh")
+    assert "tbh_contract_slices" in names and "tbh_maxplus_gemm" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.TBH_SIGNATURES), "ctypes table and header differ"
+
+
+def test_workload_exports_everything_declared(tb):
+    from tensorbranching_jl_amd import _lib
+    lib = _lib.tbw()
+    names = _declared("tbh_workload.h", "tbw")
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.TBW_SIGNATURES)
+
+
+def _slice_struct(tb, br):
+    from tensorbranching_jl_amd.api import _Marshal
+    return _Marshal([br], np.float32)
+
+
+def test_host_complexity_and_memory_match_oracle(tb, orc):
+    g = tb.random_regular_graph(80, 3, seed=11)
+    for b in tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=9, order_trials=8))[:20]:
+        if b.code is None:
+            continue
+        cc = tb.complexity(b)
+        tc, sc, rw = orc.complexity(b.code)
+        assert (cc.tc, cc.sc, cc.rwc) == pytest.approx((tc, sc, rw))
+        assert tb.contraction_peak_memory(b.code, b.p.g.nv) == pytest.approx(orc.peak_memory(b.code))
+        assert tb.contraction_all_memory(b.code, b.p.g.nv) == pytest.approx(orc.all_memory(b.code))
+
+
+def test_malformed_slices_are_rejected(tb):
+    from tensorbranching_jl_amd import _lib
+    lib = _lib.tbh()
+    ix_ptr = np.array([0, 1, 2, 4], np.int32)
+    ix = np.array([0, 1, 0, 1], np.int32)
+
+    def call(tree, nv=2, ixp=ix_ptr, ixx=ix):
+        s = _lib.TbhSlice(nv, None, len(ixp) - 1, ixp.ctypes.data, ixx.ctypes.data,
+                          tree.ctypes.data, 0.0)
+        tc, sc, rw = C.c_double(), C.c_double(), C.c_double()
+        return lib.tbh_complexity(C.byref(s), C.byref(tc), C.byref(sc), C.byref(rw))
+
+    assert call(np.array([0, 2, 3, 1], np.int32)) == 0          # ((0,2),1)... valid
+    assert call(np.array([0, 0, 3, 1], np.int32)) == _lib.TBH_EINVAL  # leaf used twice
+    assert b"malformed" in lib.tbh_last_error()
+    assert call(np.array([0, 4, 3, 1], np.int32)) == _lib.TBH_EINVAL  # forward reference
+    bad_ix = np.array([0, 1, 0, 5], np.int32)
+    assert call(np.array([0, 2, 3, 1], np.int32), ixx=bad_ix) == _lib.TBH_EINVAL
+    assert b"out of range" in lib.tbh_last_error()
+    rank3 = np.array([0, 1, 2, 5], np.int32)
+    assert call(np.array([0, 2, 3, 1], np.int32), ixp=rank3,
+                ixx=np.array([0, 1, 0, 1, 1], np.int32)) == _lib.TBH_EINVAL
+
+
+def test_plan_steps_cover_every_node(tb):
+    g = tb.random_regular_graph(60, 3, seed=2)
+    b = tb.SlicedBranch(tb.MISProblem(g), tb.initialize_code(g, 8), 0)
+    nodes = b.code.nleaves - 1
+    plain = tb.plan_steps(b, mode="plain")
+    batched = tb.plan_steps(b, mode="batched")
+    # every tree node is at least one step; batched never adds OMEinsum permutes
+    assert len(plain) >= nodes and len(batched) >= nodes
+    assert len(batched) <= len(plain)
+    assert max(s["rank_c"] for s in batched) <= tb.complexity(b).sc + 1
+    levels = [s["level"] for s in batched]
+    assert min(levels) >= 1
+    assert [s["level"] for s in plain] == list(range(1, len(plain) + 1))
+
+
+def test_footprint_bounds_reference_model(tb):
+    """The batched arena holds at least the largest tensor and is of the
+    order of the reference's peak-memory model."""
+    g = tb.random_regular_graph(70, 3, seed=6)
+    b = tb.SlicedBranch(tb.MISProblem(g), tb.initialize_code(g, 8), 0)
+    fp = tb.slice_footprint(b)
+    peak = 2 ** tb.contraction_peak_memory(b.code, g.nv) * 4
+    assert fp >= 4 * 2 ** tb.complexity(b).sc
+    assert fp <= 64 * peak

@@ -16,6 +16,14 @@
 
 #include "tbh_internal.hpp"
 
+// minimum waves per SIMD the EW kernels are register-limited to
+#ifndef TBH_EW_WAVES
+#define TBH_EW_WAVES 4
+#endif
+#ifndef TBH_DIRECT_WAVES
+#define TBH_DIRECT_WAVES 4
+#endif
+
 namespace tbh {
 
 template <typename T>
@@ -94,53 +102,63 @@ __device__ __forceinline__ void ew_thread_setup(const EwDesc* __restrict__ d, Ew
   hi_strides(e.hTC, d->tC);
 }
 
-// One tile: outer configuration o of step d.
-template <typename T, bool HASB>
-__device__ __forceinline__ void ew_tile(const EwDesc* __restrict__ d, uint32_t o,
-                                        const EwThread& e, T* __restrict__ lA,
-                                        T* __restrict__ lB) {
-  const int tid = threadIdx.x;
-  uint32_t offA = 0, offB = 0, offC = 0;
+// outer offsets of tile o (sum of the strides of its set bits)
+struct EwOuter {
+  uint32_t a, b, c;
+};
+__device__ __forceinline__ EwOuter outer_offsets(const EwDesc* __restrict__ d, uint32_t o) {
+  EwOuter r{0, 0, 0};
   const int nO = d->nO;
 #pragma unroll
   for (int k = 0; k < 32; k++)
     if (k < nO && ((o >> k) & 1)) {
-      offA += pos_stride(d->oA[k]);
-      offB += pos_stride(d->oB[k]);
-      offC += pos_stride(d->oC[k]);
+      r.a += pos_stride(d->oA[k]);
+      r.b += pos_stride(d->oB[k]);
+      r.c += pos_stride(d->oC[k]);
     }
-  const T* __restrict__ A = reinterpret_cast<const T*>(d->a) + offA;
-  const T* __restrict__ B = reinterpret_cast<const T*>(d->b) + offB;
-  T* __restrict__ C = reinterpret_cast<T*>(d->c) + offC;
-  // 1. the tile's sub-tensors of A and B into LDS (contiguous reads)
-  {
-    const int nA = 1 << d->nSA;
-    T v[16];
-#pragma unroll
-    for (int m = 0; m < 16; m++)
-      if (tid + 256 * m < nA) v[m] = A[e.loSA + hi_at(e.hSA, m)];
-#pragma unroll
-    for (int m = 0; m < 16; m++)
-      if (tid + 256 * m < nA) lA[lds_pad(tid + 256 * m)] = v[m];
+  return r;
+}
+// from tile o-1 to tile o: only the bits that flip change
+__device__ __forceinline__ void outer_step(const EwDesc* __restrict__ d, uint32_t o, EwOuter& r) {
+  uint32_t flip = o ^ (o - 1);
+  while (flip) {
+    const int k = __builtin_ctz(flip);
+    flip &= flip - 1;
+    const uint32_t sa = pos_stride(d->oA[k]), sb = pos_stride(d->oB[k]), sc = pos_stride(d->oC[k]);
+    if ((o >> k) & 1) {
+      r.a += sa;
+      r.b += sb;
+      r.c += sc;
+    } else {
+      r.a -= sa;
+      r.b -= sb;
+      r.c -= sc;
+    }
   }
-  if (HASB) {
-    const int nB = 1 << d->nSB;
-    T v[16];
+}
+
+// Reduce phase of a tile with CNT = max(1, 2^nT / 256) outputs per thread.
+// A sub-tensor index is a sum of distinct powers of two (tile bits and
+// reduced bits sit at different positions), and lds_pad(x + y) =
+// lds_pad(x) + lds_pad(y) when x and y share no bit, so the padded address of
+// (output m, reduced configuration) is a per-thread part plus a wave-uniform
+// part: one add per LDS read.
+template <typename T, bool HASB, int CNT>
+__device__ __forceinline__ void ew_tile_reduce(const EwDesc* __restrict__ d, const EwThread& e,
+                                               const T* __restrict__ lA,
+                                               const T* __restrict__ lB, T* __restrict__ C) {
+  const int tid = threadIdx.x;
+  if (CNT == 1 && tid >= (1 << d->nT)) return;
+  uint32_t pA[CNT], pB[CNT];
+  T acc[CNT];
 #pragma unroll
-    for (int m = 0; m < 16; m++)
-      if (tid + 256 * m < nB) v[m] = B[e.loSB + hi_at(e.hSB, m)];
-#pragma unroll
-    for (int m = 0; m < 16; m++)
-      if (tid + 256 * m < nB) lB[lds_pad(tid + 256 * m)] = v[m];
+  for (int m = 0; m < CNT; m++) {
+    acc[m] = Trop<T>::zero();
+    pA[m] = lds_pad(e.loTA + hi_at(e.hTA, m));
+    if (HASB) pB[m] = lds_pad(e.loTB + hi_at(e.hTB, m));
   }
-  __syncthreads();
-  // 2. max over the reduced configurations: the lowest (up to 3) reduced
-  //    bits are unrolled with their strides in registers, taken in pairs
-  //    (add + max3); the rest walk a Gray code, one stride change each
-  const int nOut = 1 << d->nT;
-  T acc[16];
-#pragma unroll
-  for (int m = 0; m < 16; m++) acc[m] = Trop<T>::zero();
+  // the lowest (up to 3) reduced bits unrolled in pairs (add + max3), the
+  // rest walked in Gray-code order, one stride change per step
   const int nR = d->nR;
   const int nlo = nR < 3 ? nR : 3;
   uint32_t lsA[4] = {0, 0, 0, 0}, lsB[4] = {0, 0, 0, 0};
@@ -169,29 +187,74 @@ __device__ __forceinline__ void ew_tile(const EwDesc* __restrict__ d, uint32_t o
 #pragma unroll
     for (int q = 0; q < 8; q += 2) {
       if (q >= nq) break;
-      const uint32_t qa0 = rA + hi_at(lsA, q), qb0 = rB + hi_at(lsB, q);
-      const uint32_t qa1 = rA + hi_at(lsA, q + 1), qb1 = rB + hi_at(lsB, q + 1);
+      const uint32_t qa0 = lds_pad(rA + hi_at(lsA, q)), qb0 = lds_pad(rB + hi_at(lsB, q));
+      if (nq > 1) {
+        const uint32_t qa1 = lds_pad(rA + hi_at(lsA, q + 1));
+        const uint32_t qb1 = lds_pad(rB + hi_at(lsB, q + 1));
 #pragma unroll
-      for (int m = 0; m < 16; m++) {
-        if (tid + 256 * m < nOut) {
-          const uint32_t ia = e.loTA + hi_at(e.hTA, m), ib = e.loTB + hi_at(e.hTB, m);
-          T x0 = lA[lds_pad(ia + qa0)];
-          if (HASB) x0 = Trop<T>::mul(x0, lB[lds_pad(ib + qb0)]);
-          if (nq > 1) {
-            T x1 = lA[lds_pad(ia + qa1)];
-            if (HASB) x1 = Trop<T>::mul(x1, lB[lds_pad(ib + qb1)]);
-            acc[m] = Trop<T>::add(acc[m], Trop<T>::add(x0, x1));
-          } else {
-            acc[m] = Trop<T>::add(acc[m], x0);
+        for (int m = 0; m < CNT; m++) {
+          T x0 = lA[pA[m] + qa0], x1 = lA[pA[m] + qa1];
+          if (HASB) {
+            x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
+            x1 = Trop<T>::mul(x1, lB[pB[m] + qb1]);
           }
+          acc[m] = Trop<T>::add(acc[m], Trop<T>::add(x0, x1));
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < CNT; m++) {
+          T x0 = lA[pA[m] + qa0];
+          if (HASB) x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
+          acc[m] = Trop<T>::add(acc[m], x0);
         }
       }
     }
   }
-  // 3. contiguous writes along C's lowest positions
+  // contiguous writes along C's lowest positions
+#pragma unroll
+  for (int m = 0; m < CNT; m++) C[e.loTC + hi_at(e.hTC, m)] = Trop<T>::canon(acc[m]);
+}
+
+// the sub-tensor of one operand (2^nS elements) into LDS; lds_pad(tid + 256m)
+// = lds_pad(tid) + 264m
+template <typename T>
+__device__ __forceinline__ void ew_stage(const T* __restrict__ X, int nS, uint32_t lo,
+                                         const uint32_t (&hS)[4], T* __restrict__ lX) {
+  const int tid = threadIdx.x;
+  const int n = 1 << nS;
+  const uint32_t pt = lds_pad(tid);
+  if (n <= 256) {
+    if (tid < n) lX[pt] = X[lo];
+    return;
+  }
+  const int cnt = n >> 8;  // 2..16, wave-uniform
+  T v[16];
 #pragma unroll
   for (int m = 0; m < 16; m++)
-    if (tid + 256 * m < nOut) C[e.loTC + hi_at(e.hTC, m)] = Trop<T>::canon(acc[m]);
+    if (m < cnt) v[m] = X[lo + hi_at(hS, m)];
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    if (m < cnt) lX[pt + 264 * m] = v[m];
+}
+
+// One tile of step d at outer offsets off.
+template <typename T, bool HASB>
+__device__ __forceinline__ void ew_tile(const EwDesc* __restrict__ d, const EwOuter& off,
+                                        const EwThread& e, T* __restrict__ lA,
+                                        T* __restrict__ lB) {
+  const T* __restrict__ A = reinterpret_cast<const T*>(d->a) + off.a;
+  const T* __restrict__ B = reinterpret_cast<const T*>(d->b) + off.b;
+  T* __restrict__ C = reinterpret_cast<T*>(d->c) + off.c;
+  ew_stage<T>(A, d->nSA, e.loSA, e.hSA, lA);
+  if (HASB) ew_stage<T>(B, d->nSB, e.loSB, e.hSB, lB);
+  __syncthreads();
+  switch (d->nT <= 8 ? 0 : d->nT - 8) {
+    case 0: ew_tile_reduce<T, HASB, 1>(d, e, lA, lB, C); break;
+    case 1: ew_tile_reduce<T, HASB, 2>(d, e, lA, lB, C); break;
+    case 2: ew_tile_reduce<T, HASB, 4>(d, e, lA, lB, C); break;
+    case 3: ew_tile_reduce<T, HASB, 8>(d, e, lA, lB, C); break;
+    default: ew_tile_reduce<T, HASB, 16>(d, e, lA, lB, C); break;
+  }
   __syncthreads();
 }
 
@@ -229,73 +292,42 @@ __device__ __forceinline__ void ew_lane_setup(const EwDesc* __restrict__ d, int 
   }
 }
 
-// outer offsets of tile o (sum of the strides of its set bits)
-struct EwOuter {
-  uint32_t a, b, c;
-};
-__device__ __forceinline__ EwOuter outer_offsets(const EwDesc* __restrict__ d, uint32_t o) {
-  EwOuter r{0, 0, 0};
-  const int nO = d->nO;
-#pragma unroll
-  for (int k = 0; k < 32; k++)
-    if (k < nO && ((o >> k) & 1)) {
-      r.a += pos_stride(d->oA[k]);
-      r.b += pos_stride(d->oB[k]);
-      r.c += pos_stride(d->oC[k]);
-    }
-  return r;
-}
-// from tile o-1 to tile o: only the bits that flip change
-__device__ __forceinline__ void outer_step(const EwDesc* __restrict__ d, uint32_t o, EwOuter& r) {
-  uint32_t flip = o ^ (o - 1);
-  while (flip) {
-    const int k = __builtin_ctz(flip);
-    flip &= flip - 1;
-    const uint32_t sa = pos_stride(d->oA[k]), sb = pos_stride(d->oB[k]), sc = pos_stride(d->oC[k]);
-    if ((o >> k) & 1) {
-      r.a += sa;
-      r.b += sb;
-      r.c += sc;
-    } else {
-      r.a -= sa;
-      r.b -= sb;
-      r.c -= sc;
-    }
-  }
-}
-
-template <typename T, bool HASB>
+// CNT = max(1, 2^nT / 64) outputs per lane (nT < 6: the low 2^nT lanes).
+template <typename T, bool HASB, int CNT>
 __device__ __forceinline__ void ew_direct_tile(const EwDesc* __restrict__ d, const EwLane& L,
                                                const EwOuter& off, int lane) {
+  if (CNT == 1 && lane >= (1 << d->nT)) return;
   const T* __restrict__ A = reinterpret_cast<const T*>(d->a) + off.a;
   const T* __restrict__ B = reinterpret_cast<const T*>(d->b) + off.b;
   T* __restrict__ C = reinterpret_cast<T*>(d->c) + off.c;
-  const int nOut = 1 << d->nT, nR = d->nR;
-  T acc[16];
+  const int nR = d->nR;
+  uint32_t ia[CNT], ib[CNT];
+  T acc[CNT];
 #pragma unroll
-  for (int m = 0; m < 16; m++) acc[m] = Trop<T>::zero();
+  for (int m = 0; m < CNT; m++) {
+    ia[m] = L.la + hi_at(L.hA, m);
+    ib[m] = L.lb + hi_at(L.hB, m);
+  }
   if (nR <= 1) {
     // the common spine step: both reduced configurations' loads in flight
     // before any is consumed
     const uint32_t sA = nR ? sub_stride(d->sA, d->rA[0]) : 0u;
     const uint32_t sB = nR ? sub_stride(d->sB, d->rB[0]) : 0u;
-    T x0[16], x1[16];
+    T x0[CNT], x1[CNT];
 #pragma unroll
-    for (int m = 0; m < 16; m++) {
-      if (lane + 64 * m < nOut) {
-        const uint32_t ia = L.la + hi_at(L.hA, m), ib = L.lb + hi_at(L.hB, m);
-        x0[m] = A[ia];
-        if (nR) x1[m] = A[ia + sA];
-        if (HASB) {
-          x0[m] = Trop<T>::mul(x0[m], B[ib]);
-          if (nR) x1[m] = Trop<T>::mul(x1[m], B[ib + sB]);
-        }
+    for (int m = 0; m < CNT; m++) {
+      x0[m] = A[ia[m]];
+      if (nR) x1[m] = A[ia[m] + sA];
+      if (HASB) {
+        x0[m] = Trop<T>::mul(x0[m], B[ib[m]]);
+        if (nR) x1[m] = Trop<T>::mul(x1[m], B[ib[m] + sB]);
       }
     }
 #pragma unroll
-    for (int m = 0; m < 16; m++)
-      if (lane + 64 * m < nOut) acc[m] = nR ? Trop<T>::add(x0[m], x1[m]) : x0[m];
+    for (int m = 0; m < CNT; m++) acc[m] = nR ? Trop<T>::add(x0[m], x1[m]) : x0[m];
   } else {
+#pragma unroll
+    for (int m = 0; m < CNT; m++) acc[m] = Trop<T>::zero();
     uint32_t rA = 0, rB = 0;
     const uint32_t nr = 1u << nR;
     for (uint32_t r = 0; r < nr; r++) {
@@ -312,23 +344,32 @@ __device__ __forceinline__ void ew_direct_tile(const EwDesc* __restrict__ d, con
         }
       }
 #pragma unroll
-      for (int m = 0; m < 16; m++) {
-        if (lane + 64 * m < nOut) {
-          T x = A[L.la + hi_at(L.hA, m) + rA];
-          if (HASB) x = Trop<T>::mul(x, B[L.lb + hi_at(L.hB, m) + rB]);
-          acc[m] = Trop<T>::add(acc[m], x);
-        }
+      for (int m = 0; m < CNT; m++) {
+        T x = A[ia[m] + rA];
+        if (HASB) x = Trop<T>::mul(x, B[ib[m] + rB]);
+        acc[m] = Trop<T>::add(acc[m], x);
       }
     }
   }
 #pragma unroll
-  for (int m = 0; m < 16; m++)
-    if (lane + 64 * m < nOut) C[L.lc + hi_at(L.hC, m)] = Trop<T>::canon(acc[m]);
+  for (int m = 0; m < CNT; m++) C[L.lc + hi_at(L.hC, m)] = Trop<T>::canon(acc[m]);
+}
+
+template <typename T, bool HASB>
+__device__ __forceinline__ void ew_direct_dispatch(const EwDesc* __restrict__ d, const EwLane& L,
+                                                   const EwOuter& off, int lane) {
+  switch (d->nT <= 6 ? 0 : d->nT - 6) {
+    case 0: ew_direct_tile<T, HASB, 1>(d, L, off, lane); break;
+    case 1: ew_direct_tile<T, HASB, 2>(d, L, off, lane); break;
+    case 2: ew_direct_tile<T, HASB, 4>(d, L, off, lane); break;
+    case 3: ew_direct_tile<T, HASB, 8>(d, L, off, lane); break;
+    default: ew_direct_tile<T, HASB, 16>(d, L, off, lane); break;
+  }
 }
 
 // Persistent over wave tiles: prefix[i] is the first tile of step i.
 template <typename T>
-__global__ __launch_bounds__(256) void tbh_ew_direct_kernel(const EwDesc* __restrict__ ops,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBH_DIRECT_WAVES))) void tbh_ew_direct_kernel(const EwDesc* __restrict__ ops,
                                                             const int64_t* __restrict__ prefix,
                                                             int nops, int64_t total, int chunk) {
   const int lane = threadIdx.x & 63;
@@ -341,21 +382,29 @@ __global__ __launch_bounds__(256) void tbh_ew_direct_kernel(const EwDesc* __rest
   int64_t ob = 0, oe = 0;
   for (int64_t c0 = gw * chunk; c0 < total; c0 += nw * chunk) {
     const int64_t c1 = c0 + chunk < total ? c0 + chunk : total;
+    // one search per chunk; inside it the tiles (and steps) only move forward
+    {
+      int lo = 0, hi = nops;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= c0)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      op = lo;
+      ob = prefix[op];
+      oe = prefix[op + 1];
+      ew_lane_setup(ops + op, lane, L);
+    }
     for (int64_t t = c0; t < c1; t++) {
-      const bool fresh = t < ob || t >= oe || t == c0;
-      if (t < ob || t >= oe) {
-        int lo = 0, hi = nops;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (prefix[mid] <= t)
-            lo = mid;
-          else
-            hi = mid;
-        }
-        op = lo;
-        ob = prefix[op];
+      bool fresh = t == c0;
+      while (t >= oe) {
+        op++;
+        ob = oe;
         oe = prefix[op + 1];
-        ew_lane_setup(ops + op, lane, L);
+        fresh = true;
+        if (t < oe) ew_lane_setup(ops + op, lane, L);
       }
       const EwDesc* d = ops + op;
       const uint32_t o = (uint32_t)(t - ob);
@@ -364,9 +413,9 @@ __global__ __launch_bounds__(256) void tbh_ew_direct_kernel(const EwDesc* __rest
       else
         outer_step(d, o, off);
       if (d->b != 0)
-        ew_direct_tile<T, true>(d, L, off, lane);
+        ew_direct_dispatch<T, true>(d, L, off, lane);
       else
-        ew_direct_tile<T, false>(d, L, off, lane);
+        ew_direct_dispatch<T, false>(d, L, off, lane);
     }
   }
 }
@@ -374,7 +423,7 @@ __global__ __launch_bounds__(256) void tbh_ew_direct_kernel(const EwDesc* __rest
 // Persistent: each workgroup walks chunks of the global tile space; prefix[i]
 // is the first tile of step i (prefix[nops] = total).
 template <typename T>
-__global__ __launch_bounds__(kEwBlock) void tbh_ew_kernel(const EwDesc* __restrict__ ops,
+__global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW_WAVES))) void tbh_ew_kernel(const EwDesc* __restrict__ ops,
                                                           const int64_t* __restrict__ prefix,
                                                           int nops, int64_t total, int chunk) {
   __shared__ T lA[kLdsElems];
@@ -384,27 +433,41 @@ __global__ __launch_bounds__(kEwBlock) void tbh_ew_kernel(const EwDesc* __restri
   int64_t ob = 0, oe = 0;
   for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < total; c0 += (int64_t)gridDim.x * chunk) {
     const int64_t c1 = c0 + chunk < total ? c0 + chunk : total;
+    // one search per chunk; inside it the tiles (and steps) only move forward
+    {
+      int lo = 0, hi = nops;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= c0)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      op = lo;
+      ob = prefix[op];
+      oe = prefix[op + 1];
+      ew_thread_setup(ops + op, e);
+    }
+    EwOuter off{0, 0, 0};
     for (int64_t t = c0; t < c1; t++) {
-      if (t < ob || t >= oe) {
-        int lo = 0, hi = nops;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (prefix[mid] <= t)
-            lo = mid;
-          else
-            hi = mid;
-        }
-        op = lo;
-        ob = prefix[op];
+      bool fresh = t == c0;
+      while (t >= oe) {
+        op++;
+        ob = oe;
         oe = prefix[op + 1];
-        ew_thread_setup(ops + op, e);
+        fresh = true;
+        if (t < oe) ew_thread_setup(ops + op, e);
       }
       const EwDesc* d = ops + op;
       const uint32_t o = (uint32_t)(t - ob);
-      if (d->b != 0)
-        ew_tile<T, true>(d, o, e, lA, lB);
+      if (fresh)
+        off = outer_offsets(d, o);
       else
-        ew_tile<T, false>(d, o, e, lA, lB);
+        outer_step(d, o, off);
+      if (d->b != 0)
+        ew_tile<T, true>(d, off, e, lA, lB);
+      else
+        ew_tile<T, false>(d, off, e, lA, lB);
     }
   }
 }

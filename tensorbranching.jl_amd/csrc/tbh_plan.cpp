@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -398,6 +399,89 @@ int verify_plan(const SlicePlan& P, std::string* err) {
   return 0;
 }
 
+// Re-association of spine chains (batched mode only).  A node x = (Y, S)
+// whose big child Y = (Y', S') also absorbs a small side tensor is rotated to
+// x = (Y', F) with F = (S', S): the small side tensors are combined first and
+// the big spine tensor Y' is read once instead of materialising Y.  F keeps
+// the labels of S' U S that occur in Y' or outside x, so x's own labels are
+// unchanged.  Accepted while F stays small and x reduces few labels.  The
+// result is the same: the tropical contraction is order-invariant (values are
+// exact), which the tests check against the unmodified tree.
+void reassociate(int32_t nl, int32_t nv, std::vector<int32_t>& tree,
+                 const std::vector<std::vector<int32_t>>& lab0, const PlanOptions& opt) {
+  const int32_t nn = nl - 1;
+  if (nn < 3) return;
+  const int W = (nv + 63) / 64;
+  const int tot = nl + nn;
+  std::vector<uint64_t> L((size_t)tot * W, 0);
+  auto Lp = [&](int x) { return &L[(size_t)x * W]; };
+  for (int x = 0; x < tot; x++)
+    for (int32_t l : lab0[x]) Lp(x)[l >> 6] |= 1ull << (l & 63);
+  auto rank = [&](int x) {
+    int c = 0;
+    for (int i = 0; i < W; i++) c += __builtin_popcountll(Lp(x)[i]);
+    return c;
+  };
+  std::vector<int32_t> lc(tot, -1), rc(tot, -1), rk(tot, 0);
+  for (int x = 0; x < tot; x++) rk[x] = rank(x);
+  for (int32_t j = 0; j < nn; j++) {
+    lc[nl + j] = tree[2 * j];
+    rc[nl + j] = tree[2 * j + 1];
+  }
+  std::vector<uint64_t> F(W);
+  // a combined side tensor must never count as big (F <= side < big), so each
+  // rotation moves one node down the spine and the loop ends
+  const int side = std::min(opt.chain_side, opt.chain_big - 1);
+  for (int32_t j = 0; j < nn; j++) {
+    const int x = nl + j;
+    for (int guard = 0; guard < nn; guard++) {
+      int Y = rk[lc[x]] >= rk[rc[x]] ? lc[x] : rc[x];
+      int S = Y == lc[x] ? rc[x] : lc[x];
+      if (Y < nl || rk[Y] < opt.chain_big || rk[S] > side) break;
+      int Yp = rk[lc[Y]] >= rk[rc[Y]] ? lc[Y] : rc[Y];
+      int Sp = Yp == lc[Y] ? rc[Y] : lc[Y];
+      if (rk[Yp] < opt.chain_big || rk[Sp] > side) break;
+      int rf = 0, nred = 0;
+      for (int i = 0; i < W; i++) {
+        F[i] = (Lp(Sp)[i] | Lp(S)[i]) & (Lp(Yp)[i] | Lp(x)[i]);
+        rf += __builtin_popcountll(F[i]);
+        nred += __builtin_popcountll((Lp(Yp)[i] | F[i]) & ~Lp(x)[i]);
+      }
+      if (rf > side || nred > opt.chain_reduce) break;
+      // node Y becomes F = (S', S); x = (Y', F)
+      lc[Y] = Sp;
+      rc[Y] = S;
+      std::copy(F.begin(), F.end(), Lp(Y));
+      rk[Y] = rf;
+      lc[x] = Yp;
+      rc[x] = Y;
+    }
+  }
+  // serialise in post-order again
+  std::vector<int32_t> id(tot, -1);
+  for (int i = 0; i < nl; i++) id[i] = i;
+  std::vector<int32_t> out;
+  out.reserve(2 * nn);
+  int32_t next = nl;
+  std::vector<std::pair<int32_t, int>> st;
+  st.emplace_back(nl + nn - 1, 0);
+  while (!st.empty()) {
+    auto [x, state] = st.back();
+    st.pop_back();
+    if (x < nl) continue;
+    if (state == 0) {
+      st.emplace_back(x, 1);
+      st.emplace_back(rc[x], 0);
+      st.emplace_back(lc[x], 0);
+    } else {
+      id[x] = next++;
+      out.push_back(id[lc[x]]);
+      out.push_back(id[rc[x]]);
+    }
+  }
+  tree.swap(out);
+}
+
 }  // namespace
 
 int slice_labels(const SliceIn& s, std::vector<std::vector<int32_t>>* labels, std::string* err) {
@@ -468,18 +552,37 @@ int slice_peak_memory(const SliceIn& s, double* peak, double* all, std::string* 
   return 0;
 }
 
-int plan_slice(const SliceIn& s, const PlanOptions& opt, SlicePlan* out, std::string* err) {
+int plan_slice(const SliceIn& s_in, const PlanOptions& opt, SlicePlan* out, std::string* err) {
   SlicePlan& P = *out;
   P = SlicePlan();
-  if (validate(s, err)) return -1;
-  if (s.nv == 0 || s.nleaves == 0) {
+  if (validate(s_in, err)) return -1;
+  if (s_in.nv == 0 || s_in.nleaves == 0) {
     P.empty = true;
     return 0;
   }
   P.empty = false;
+  SliceIn s = s_in;
   const int32_t nl = s.nleaves, nn = nl - 1;
   std::vector<std::vector<int32_t>> lab;
   compute_labels(s, lab);
+  // tree-node contraction count of the given tree (the reference's tc)
+  for (int32_t j = 0; j < nn; j++) {
+    const auto &A = lab[s.tree[2 * j]], &Bv = lab[s.tree[2 * j + 1]];
+    int u = (int)A.size();
+    for (int32_t l : Bv)
+      if (pos_in(A, l) < 0) u++;
+    P.node_macs += std::ldexp(1.0, u);
+  }
+  std::vector<int32_t> tree2;
+  if (opt.batched && opt.reassociate && nn >= 3 && !std::getenv("TBH_NO_REASSOCIATE")) {
+    tree2.assign(s.tree, s.tree + 2 * (size_t)nn);
+    PlanOptions o2 = opt;
+    if (const char* e = std::getenv("TBH_CHAIN"))  // "big,side,reduce" (tuning sweeps)
+      std::sscanf(e, "%d,%d,%d", &o2.chain_big, &o2.chain_side, &o2.chain_reduce);
+    reassociate(nl, s.nv, tree2, lab, o2);
+    s.tree = tree2.data();
+    compute_labels(s, lab);
+  }
 
   // elimination step of each label (node index where it disappears)
   std::vector<int32_t> elim(s.nv, nn);
@@ -561,15 +664,6 @@ int plan_slice(const SliceIn& s, const PlanOptions& opt, SlicePlan* out, std::st
     for (auto* p : parts) r.insert(r.end(), p->begin(), p->end());
     return r;
   };
-
-  // tree-node contraction count (the reference's tc)
-  for (int32_t j = 0; j < nn; j++) {
-    const auto &A = lab[s.tree[2 * j]], &Bv = lab[s.tree[2 * j + 1]];
-    int u = (int)A.size();
-    for (int32_t l : Bv)
-      if (pos_in(A, l) < 0) u++;
-    P.node_macs += std::ldexp(1.0, u);
-  }
 
   if (opt.batched) {
     // top-down layout requests; GEMM-shaped nodes ask their children for

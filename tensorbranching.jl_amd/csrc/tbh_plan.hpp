@@ -71,6 +71,11 @@ struct PlanOptions {
   // GEMM-shaped nodes: M, N, K (and the product of the reduction with the
   // outputs) large enough that register blocking beats streaming
   int gemm_min_m = 16, gemm_min_n = 16, gemm_min_k = 64;
+  // spine-chain re-association (batched only): spine tensors of rank >=
+  // chain_big absorbing side tensors of rank <= chain_side, combined side
+  // tensor <= chain_side, at most chain_reduce labels reduced per step
+  bool reassociate = true;
+  int chain_big = 16, chain_side = 15, chain_reduce = 5;
 };
 
 // Returns 0 or sets *err and returns -1.

@@ -319,8 +319,11 @@ int direct_grid(tbh_ctx* ctx, int64_t total, int chunk) {
   return (int)std::max<int64_t>(1, std::min<int64_t>((waves + 3) / 4, (int64_t)ctx->num_cus * 8));
 }
 
-constexpr int kChunk = 2;
-constexpr int kChunkDirect = 4;
+// Waves (direct) and workgroups (tiled) take small chunks of the tile space
+// round-robin, which spreads expensive steps evenly; long contiguous ranges
+// measured slower (costly steps cluster in a few ranges).
+int direct_chunk(tbh_ctx*, int64_t) { return 4; }
+int ew_chunk(tbh_ctx*, int64_t) { return 2; }
 
 }  // namespace
 
@@ -505,7 +508,8 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
           ev = ctx->prof.begin(ctx->stream);
           e = launch_ew_direct(dtype, (const EwDesc*)(dbase + lo[L].ews) + k,
                                (const int64_t*)(dbase + lo[L].onew) + 2 * k, 1, d.tiles,
-                               direct_grid(ctx, d.tiles, kChunkDirect), kChunkDirect, ctx->stream);
+                               direct_grid(ctx, d.tiles, direct_chunk(ctx, d.tiles)),
+                               direct_chunk(ctx, d.tiles), ctx->stream);
           char tag[128];
           std::snprintf(tag, sizeof tag, "nT=%d nO=%d nR=%d nSA=%d nSB=%d ra=%d rb=%d", d.nT,
                         d.nO, d.nR, d.nSA, d.nSB, ews_ranks[L][k].first, ews_ranks[L][k].second);
@@ -515,8 +519,9 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
         if (prof) ev = ctx->prof.begin(ctx->stream);
         e = launch_ew_direct(dtype, (const EwDesc*)(dbase + lo[L].ews),
                              (const int64_t*)(dbase + lo[L].ewsp), (int)ews[L].size(),
-                             lo[L].ews_total, direct_grid(ctx, lo[L].ews_total, kChunkDirect),
-                             kChunkDirect, ctx->stream);
+                             lo[L].ews_total,
+                             direct_grid(ctx, lo[L].ews_total, direct_chunk(ctx, lo[L].ews_total)),
+                             direct_chunk(ctx, lo[L].ews_total), ctx->stream);
         if (prof)
           ctx->prof.end(ctx->stream, ev, 0, L, (int)ews[L].size(), lvl_bytes[0][L], lvl_macs[0][L]);
         ctx->stats.launches++;
@@ -528,7 +533,8 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
           ev = ctx->prof.begin(ctx->stream);
           e = launch_ew(dtype, (const EwDesc*)(dbase + lo[L].ew) + k,
                         (const int64_t*)(dbase + lo[L].one) + 2 * k, 1, d.tiles,
-                        ew_grid(ctx, d.tiles, kChunk), kChunk, ctx->stream);
+                        ew_grid(ctx, d.tiles, ew_chunk(ctx, d.tiles)), ew_chunk(ctx, d.tiles),
+                        ctx->stream);
           char tag[128];
           std::snprintf(tag, sizeof tag, "nT=%d nO=%d nR=%d nSA=%d nSB=%d ra=%d rb=%d", d.nT,
                         d.nO, d.nR, d.nSA, d.nSB, ew_ranks[L][k].first, ew_ranks[L][k].second);
@@ -538,8 +544,9 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
       } else if (e == hipSuccess && lo[L].ew_total > 0) {
         if (prof) ev = ctx->prof.begin(ctx->stream);
         e = launch_ew(dtype, (const EwDesc*)(dbase + lo[L].ew), (const int64_t*)(dbase + lo[L].ewp),
-                      (int)ew[L].size(), lo[L].ew_total, ew_grid(ctx, lo[L].ew_total, kChunk),
-                      kChunk, ctx->stream);
+                      (int)ew[L].size(), lo[L].ew_total,
+                      ew_grid(ctx, lo[L].ew_total, ew_chunk(ctx, lo[L].ew_total)),
+                      ew_chunk(ctx, lo[L].ew_total), ctx->stream);
         if (prof)
           ctx->prof.end(ctx->stream, ev, 1, L, (int)ew[L].size(), lvl_bytes[1][L], lvl_macs[1][L]);
         ctx->stats.launches++;
@@ -651,11 +658,13 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
       if (op.direct)
         err = launch_ew_direct(dtype, (const EwDesc*)(dbase + o_op[k]),
                                (const int64_t*)(dbase + o_pre[k]), 1, tiles[k],
-                               direct_grid(ctx, tiles[k], kChunkDirect), kChunkDirect, ctx->stream);
+                               direct_grid(ctx, tiles[k], direct_chunk(ctx, tiles[k])),
+                               direct_chunk(ctx, tiles[k]), ctx->stream);
       else
         err = launch_ew(dtype, (const EwDesc*)(dbase + o_op[k]),
                         (const int64_t*)(dbase + o_pre[k]), 1, tiles[k],
-                        ew_grid(ctx, tiles[k], kChunk), kChunk, ctx->stream);
+                        ew_grid(ctx, tiles[k], ew_chunk(ctx, tiles[k])), ew_chunk(ctx, tiles[k]),
+                        ctx->stream);
       ctx->stats.ew_ops++;
       ctx->stats.ew_ops_algo_bytes += op.bytes;
     } else {

@@ -282,14 +282,31 @@ struct Builder {
   }
 };
 
-// A node runs as a tiled GEMM when M, N, K are all large, or when K is large
-// enough that streaming it per output (2 loads per MAC) loses to register
-// blocking even in a partly filled 128x128 tile.
+// A node runs as a tiled GEMM when M, N >= 64 and K >= 16, or when its
+// outputs fill at least a quarter of a 128 x 128 register-blocked tile and K
+// is long enough (>= 64) to amortise the partly idle tile; smaller outputs
+// run as EW steps (a partly filled tile still computes the whole tile).
 bool gemm_shaped(const PlanOptions& opt, int na, int nb, int nk) {
   const int64_t M = (int64_t)1 << std::min(na, 30), N = (int64_t)1 << std::min(nb, 30),
                 K = (int64_t)1 << std::min(nk, 30);
   if (M >= 64 && N >= 64 && K >= 16) return true;
-  return M >= opt.gemm_min_m && N >= opt.gemm_min_n && K >= opt.gemm_min_k;
+  return std::min<int64_t>(M, 128) * std::min<int64_t>(N, 128) >= opt.gemm_min_fill &&
+         K >= opt.gemm_min_k;
+}
+
+// Split-K for a GEMM node (batched plans): a node with few 128 x 128 tiles
+// and a long K loop keeps its s highest K labels as batch labels of a
+// temporary, reduced afterwards by an EW step, so that its tiles spread over
+// the CUs.  Each part keeps K >= 2^opt.split_min_k_bits.
+int gemm_split(const PlanOptions& opt, int na, int nb, int nk, int nt) {
+  if (!opt.split_k) return 0;
+  // log2 of the node's tile count
+  const int tile_bits = (std::max(na, 7) - 7) + (std::max(nb, 7) - 7) + nt;
+  int s = 0;
+  while (tile_bits + s < opt.split_tiles_bits && nk - s - 1 >= opt.split_min_k_bits &&
+         na + nb + nt + s + 1 <= opt.split_max_rank)
+    s++;
+  return s;
 }
 
 // Interval reuse of arena blocks: blocks freed by tensors that died at a level
@@ -552,7 +569,13 @@ int slice_peak_memory(const SliceIn& s, double* peak, double* all, std::string* 
   return 0;
 }
 
-int plan_slice(const SliceIn& s_in, const PlanOptions& opt, SlicePlan* out, std::string* err) {
+int plan_slice(const SliceIn& s_in, const PlanOptions& opt_in, SlicePlan* out, std::string* err) {
+  PlanOptions opt = opt_in;
+  // tuning overrides: TBH_CHAIN="big,side,reduce", TBH_SPLITK="tile_bits,min_k_bits,max_rank"
+  if (const char* e = std::getenv("TBH_CHAIN"))
+    std::sscanf(e, "%d,%d,%d", &opt.chain_big, &opt.chain_side, &opt.chain_reduce);
+  if (const char* e = std::getenv("TBH_SPLITK"))
+    std::sscanf(e, "%d,%d,%d", &opt.split_tiles_bits, &opt.split_min_k_bits, &opt.split_max_rank);
   SlicePlan& P = *out;
   P = SlicePlan();
   if (validate(s_in, err)) return -1;
@@ -576,10 +599,7 @@ int plan_slice(const SliceIn& s_in, const PlanOptions& opt, SlicePlan* out, std:
   std::vector<int32_t> tree2;
   if (opt.batched && opt.reassociate && nn >= 3 && !std::getenv("TBH_NO_REASSOCIATE")) {
     tree2.assign(s.tree, s.tree + 2 * (size_t)nn);
-    PlanOptions o2 = opt;
-    if (const char* e = std::getenv("TBH_CHAIN"))  // "big,side,reduce" (tuning sweeps)
-      std::sscanf(e, "%d,%d,%d", &o2.chain_big, &o2.chain_side, &o2.chain_reduce);
-    reassociate(nl, s.nv, tree2, lab, o2);
+    reassociate(nl, s.nv, tree2, lab, opt);
     s.tree = tree2.data();
     compute_labels(s, lab);
   }
@@ -672,6 +692,7 @@ int plan_slice(const SliceIn& s_in, const PlanOptions& opt, SlicePlan* out, std:
     std::vector<std::vector<int32_t>> req(nl + nn);
     std::vector<Cls> cls(nn);
     std::vector<std::vector<int32_t>> gemm_out(nn);
+    std::vector<int> split(nn, 0);
     for (int32_t j = nn - 1; j >= 0; j--) {
       cls[j] = classify(j);
       Cls& c = cls[j];
@@ -681,15 +702,21 @@ int plan_slice(const SliceIn& s_in, const PlanOptions& opt, SlicePlan* out, std:
       is_gemm[j] = g;
       if (g) {
         auto sa = sorted_key(c.a), sb = sorted_key(c.b), sk = sorted_key(c.k), st = sorted_key(c.t);
+        // split-K: the last ks K labels (the latest eliminated stay lowest)
+        // become batch labels between the outputs and t
+        const int ks = gemm_split(opt, na, nb, nk, (int)st.size());
+        std::vector<int32_t> klo(sk.begin(), sk.end() - ks), khi(sk.end() - ks, sk.end());
+        split[j] = ks;
         c.a = sa;
         c.b = sb;
         c.k = sk;
         c.t = st;
-        gemm_out[j] = concat({&sa, &sb, &st});
-        req[c.ca] = concat({&sa, &sk, &st});
-        req[c.cb] = concat({&sk, &sb, &st});
+        gemm_out[j] = concat({&sa, &sb, &khi, &st});
+        req[c.ca] = concat({&sa, &klo, &khi, &st});
+        req[c.cb] = concat({&klo, &sb, &khi, &st});
         has_req[c.ca] = has_req[c.cb] = 1;
-        B.layout[o] = has_req[o] ? req[o] : gemm_out[j];
+        const auto node_out = concat({&sa, &sb, &st});
+        B.layout[o] = has_req[o] ? req[o] : node_out;
       } else {
         B.layout[o] = has_req[o] ? req[o] : sorted_key(lab[o]);
       }
@@ -703,7 +730,8 @@ int plan_slice(const SliceIn& s_in, const PlanOptions& opt, SlicePlan* out, std:
       if (is_gemm[j]) {
         int dst = o;
         if (B.layout[o] != gemm_out[j]) dst = B.add_tensor(gemm_out[j]);
-        B.gemm(c.ca, c.cb, dst, (int)c.a.size(), (int)c.b.size(), (int)c.k.size(), (int)c.t.size());
+        B.gemm(c.ca, c.cb, dst, (int)c.a.size(), (int)c.b.size(), (int)c.k.size() - split[j],
+               (int)c.t.size() + split[j]);
         if (dst != o && B.ew(dst, -1, o)) return -1;
       } else {
         if (B.ew(s.tree[2 * j], s.tree[2 * j + 1], o)) return -1;

@@ -68,9 +68,13 @@ struct SlicePlan {
 
 struct PlanOptions {
   bool batched = true;   // wave levels + layout planning; false: OMEinsum lowering
-  // GEMM-shaped nodes: M, N, K (and the product of the reduction with the
-  // outputs) large enough that register blocking beats streaming
-  int gemm_min_m = 16, gemm_min_n = 16, gemm_min_k = 64;
+  // GEMM-shaped nodes: M, N >= 64 (K >= 16), or min(M,128) x min(N,128) >=
+  // gemm_min_fill with K >= gemm_min_k
+  int gemm_min_fill = 4096, gemm_min_k = 64;
+  // split-K of GEMM nodes (batched only): up to 2^split_tiles_bits tiles per
+  // node, K parts of >= 2^split_min_k_bits, temporaries of rank <= split_max_rank
+  bool split_k = true;
+  int split_tiles_bits = 9, split_min_k_bits = 11, split_max_rank = 27;
   // spine-chain re-association (batched only): spine tensors of rank >=
   // chain_big absorbing side tensors of rank <= chain_side, combined side
   // tensor <= chain_side, at most chain_reduce labels reduced per step

@@ -375,6 +375,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     std::vector<std::vector<std::pair<int, int>>> ew_ranks(H + 1);
     std::vector<std::vector<double>> ews_bytes(H + 1), ews_macs(H + 1);  // direct steps
     std::vector<std::vector<std::pair<int, int>>> ews_ranks(H + 1);
+    std::vector<std::vector<double>> gm_bytes(H + 1), gm_macs(H + 1);  // GEMM steps
     for (int k = 0; k < 3; k++) {
       lvl_bytes[k].assign(H + 1, 0.0);
       lvl_macs[k].assign(H + 1, 0.0);
@@ -412,6 +413,8 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
           xr[op.level].push_back({P.tensors[op.a].rank, op.b >= 0 ? P.tensors[op.b].rank : -1});
         } else {
           gm[op.level].push_back(make_gemm(op, P.tensors, b));
+          gm_bytes[op.level].push_back(op.bytes);
+          gm_macs[op.level].push_back(op.macs);
         }
         lvl_bytes[kind][op.level] += op.bytes;
         lvl_macs[kind][op.level] += op.macs;
@@ -429,7 +432,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     const size_t o_epi = pk.put(epi.data(), epi.size() * sizeof(EpiDesc));
     const size_t o_res = pk.reserve(4 * (i1 - i0));
     struct LvlOff {
-      size_t ew, ewp, ews, ewsp, gm, gmp, one, onew;
+      size_t ew, ewp, ews, ewsp, gm, gmp, one, onew, onegm;
       int64_t ew_total, ews_total, gm_total;
     };
     const bool per_step = ctx->profiling && std::getenv("TBH_PROFILE_STEPS");
@@ -447,6 +450,12 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
         }
         lo[L].one = pk.put(one.data(), one.size() * sizeof(int64_t));
         lo[L].onew = pk.put(onew.data(), onew.size() * sizeof(int64_t));
+        std::vector<int64_t> onegm;
+        for (const GemmDesc& d : gm[L]) {
+          onegm.push_back(0);
+          onegm.push_back(gemm_tiles(d));
+        }
+        lo[L].onegm = pk.put(onegm.data(), onegm.size() * sizeof(int64_t));
       }
       std::vector<int64_t> pe(ew[L].size() + 1, 0), pg(gm[L].size() + 1, 0);
       for (size_t k = 0; k < ew[L].size(); k++) pe[k + 1] = pe[k] + ew[L][k].tiles;
@@ -551,7 +560,19 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
           ctx->prof.end(ctx->stream, ev, 1, L, (int)ew[L].size(), lvl_bytes[1][L], lvl_macs[1][L]);
         ctx->stats.launches++;
       }
-      if (e == hipSuccess && lo[L].gm_total > 0) {
+      if (e == hipSuccess && lo[L].gm_total > 0 && per_step) {
+        for (size_t k = 0; k < gm[L].size() && e == hipSuccess; k++) {
+          const GemmDesc& d = gm[L][k];
+          ev = ctx->prof.begin(ctx->stream);
+          e = launch_gemm(dtype, true, (const GemmDesc*)(dbase + lo[L].gm) + k,
+                          (const int64_t*)(dbase + lo[L].onegm) + 2 * k, 1, gemm_tiles(d),
+                          gemm_grid(ctx, gemm_tiles(d)), ctx->stream);
+          char tag[128];
+          std::snprintf(tag, sizeof tag, "M=%d N=%d K=%d batch=%d", d.M, d.N, d.K, d.batch);
+          ctx->prof.end(ctx->stream, ev, 2, L, 1, gm_bytes[L][k], gm_macs[L][k], tag);
+          ctx->stats.launches++;
+        }
+      } else if (e == hipSuccess && lo[L].gm_total > 0) {
         if (prof) ev = ctx->prof.begin(ctx->stream);
         e = launch_gemm(dtype, true, (const GemmDesc*)(dbase + lo[L].gm),
                         (const int64_t*)(dbase + lo[L].gmp), (int)gm[L].size(), lo[L].gm_total,

@@ -334,8 +334,10 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
   size_t i0 = 0;
   while (i0 < n) {
     // admit slices while their arenas fit (leave room for descriptors)
+    // (descriptors are counted per slice below; the reserve covers the
+    // pool's first-fit fragmentation: 1/16 of the free block, <= 64 MiB)
     size_t budget = ctx->pool.largest_free();
-    budget = budget > ((size_t)64 << 20) ? budget - ((size_t)64 << 20) : 0;
+    budget -= std::min<size_t>((size_t)64 << 20, budget / 16);
     size_t i1 = i0, arena = 0, ndesc_bytes = 0;
     while (i1 < n) {
       const SlicePlan& P = *plans[i1];

@@ -93,6 +93,7 @@ struct tbh_ctx {
   int plan_threads = 8;
   bool profiling = false;
   tbh::Profiler prof;
+  std::mutex serve_mu;   // guards `serve` (never held across an execution)
   std::shared_ptr<tbh::ServeState> serve;  // created by tbh_serve_start
 };
 

@@ -29,16 +29,22 @@ class Server:
         return t.value
 
     def poll(self, ticket):
-        npdt = self._dt[ticket]
+        """(status, value, t_submit, t_done); a finished request is reported
+        once, after which its ticket is unknown."""
+        npdt = self._dt.get(ticket, np.float32)
         out = np.zeros(1, npdt)
         st = C.c_int()
         t0, t1 = C.c_double(), C.c_double()
         check(_lib.tbh().tbh_poll(self.be.handle, ticket, out.ctypes.data, C.byref(st),
                                   C.byref(t0), C.byref(t1)))
-        return _lib.STATUS[st.value], out[0], t0.value, t1.value
+        status = _lib.STATUS[st.value]
+        if status in ("done", "failed"):
+            self._dt.pop(ticket, None)
+        return status, out[0], t0.value, t1.value
 
     def wait(self, ticket):
-        npdt = self._dt[ticket]
+        """Blocks until the request finishes; returns its value or raises."""
+        npdt = self._dt.pop(ticket, np.float32)
         out = np.zeros(1, npdt)
         st = C.c_int()
         check(_lib.tbh().tbh_wait(self.be.handle, ticket, out.ctypes.data, C.byref(st)))

@@ -100,3 +100,18 @@ def test_footprint_bounds_reference_model(tb):
     peak = 2 ** tb.contraction_peak_memory(b.code, g.nv) * 4
     assert fp >= 4 * 2 ** tb.complexity(b).sc
     assert fp <= 64 * peak
+
+
+def test_serving_entry_points_reject_null_context(tb):
+    """Serving calls validate their arguments before touching a device."""
+    from tensorbranching_jl_amd import _lib
+    lib = _lib.tbh()
+    t = C.c_uint64()
+    st = C.c_int()
+    assert lib.tbh_serve_start(None, 0) == _lib.TBH_EINVAL
+    assert lib.tbh_serve_stop(None) == 0
+    assert lib.tbh_submit(None, None, C.byref(t)) == _lib.TBH_EINVAL
+    assert lib.tbh_poll(None, 1, None, C.byref(st), None, None) == _lib.TBH_EINVAL
+    assert lib.tbh_wait(None, 1, None, C.byref(st)) == _lib.TBH_EINVAL
+    assert lib.tbh_cancel(None, 1) == _lib.TBH_EINVAL
+    assert b"context" in lib.tbh_last_error()

@@ -1,0 +1,140 @@
+"""Continuous-batching scheduler (tbh_serve_start / tbh_submit / tbh_poll /
+tbh_wait / tbh_cancel): request values against the oracle and exact MIS, the
+same value served alone and mixed with other requests, per-request errors."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _sliced(tb, g, sc_target, weights=None, seed=0):
+    return tb.slice_graph(g, weights, tb.ContractionTreeSlicer(sc_target=sc_target,
+                                                               order_trials=20), seed=seed)
+
+
+def _requests(tb, orc):
+    """(branches, dtype, expected MIS) for a mix of graphs and sizes."""
+    out = []
+    for n, seed, sc in [(40, 11, 6), (60, 12, 8), (100, 0, 10), (80, 13, 12)]:
+        g = tb.random_regular_graph(n, 3, seed=seed)
+        out.append((_sliced(tb, g, sc, seed=seed), np.float32, orc.mis_exact(g)))
+    g = tb.random_ksg(7, 7, 0.8, seed=3)
+    w = np.random.default_rng(3).integers(1, 101, g.nv).astype(np.int32)
+    br = _sliced(tb, g, 8, weights=w)
+    out.append((br, np.int32, int(orc.contract_slices(br, np.int32).max())))
+    return out
+
+
+@pytest.fixture
+def be(tb):
+    b = tb.HIPBackend(0, pool_bytes=2 << 30)
+    yield b
+    b.close()
+
+
+def test_requests_match_oracle(tb, orc, be):
+    reqs = _requests(tb, orc)
+    with tb.Server(be) as srv:
+        tickets = [srv.submit(br, dt) for br, dt, _ in reqs]
+        for t, (br, dt, want) in zip(tickets, reqs):
+            got = srv.wait(t)
+            assert got == want, (t, got, want)
+            assert np.asarray(got).dtype == dt
+
+
+@pytest.mark.parametrize("cap", [0, 7, 64])
+def test_alone_equals_mixed(tb, orc, be, cap):
+    """A request's value is identical served alone and mixed into rounds with
+    other requests (small caps split its slices over several rounds)."""
+    reqs = _requests(tb, orc)
+    alone = []
+    with tb.Server(be, max_resident_slices=cap) as srv:
+        for br, dt, _ in reqs:
+            alone.append(srv.wait(srv.submit(br, dt)))
+        # burst: every request in flight at once, twice over
+        tickets = [srv.submit(br, dt) for _ in range(2) for br, dt, _ in reqs]
+        mixed = [srv.wait(t) for t in tickets]
+    for i, (br, dt, want) in enumerate(reqs):
+        assert alone[i] == want
+        assert mixed[i] == alone[i] and mixed[i + len(reqs)] == alone[i]
+    # and equal to the synchronous batched call
+    for (br, dt, _), a in zip(reqs, alone):
+        assert be.contract_slices(br, dt, "batched").max() == a
+
+
+def test_poll_reports_once_with_times(tb, orc, be):
+    g = tb.random_regular_graph(60, 3, seed=5)
+    br = _sliced(tb, g, 8, seed=5)
+    with tb.Server(be) as srv:
+        t = srv.submit(br)
+        for _ in range(2000):
+            st, val, t0, t1 = srv.poll(t)
+            if st in ("done", "failed"):
+                break
+            assert st in ("queued", "running")
+            time.sleep(0.005)
+        assert st == "done" and val == orc.mis_exact(g)
+        assert t1 >= t0 > 0
+        with pytest.raises(tb.TbhError):
+            srv.poll(t)  # a finished request is reported once
+
+
+def test_cancel_queued(tb, be):
+    """cancel drops a request that has not been admitted; an admitted one
+    refuses with TBH_EBUSY."""
+    from tensorbranching_jl_amd import _lib
+    g = tb.random_regular_graph(100, 3, seed=6)
+    big = _sliced(tb, g, 10, seed=6)
+    small = _sliced(tb, tb.random_regular_graph(30, 3, seed=7), 6, seed=7)
+    with tb.Server(be, max_resident_slices=1) as srv:
+        t_big = srv.submit(big)
+        t_small = srv.submit(small)
+        try:
+            srv.cancel(t_small)
+            cancelled = True
+        except tb.TbhError as e:
+            assert e.code == _lib.TBH_EBUSY
+            cancelled = False
+        if cancelled:
+            with pytest.raises(tb.TbhError):
+                srv.poll(t_small)
+        else:
+            srv.wait(t_small)
+        srv.wait(t_big)
+
+
+def test_failed_request_does_not_block_others(tb, orc):
+    """A slice whose arena exceeds the pool fails its request; requests in
+    other rounds still complete."""
+    be = tb.HIPBackend(0, pool_bytes=64 << 20)
+    try:
+        g_big = tb.random_regular_graph(200, 3, seed=8)  # unsliced: sc about 24
+        code = tb.initialize_code(g_big, order_trials=10, seed=8)
+        too_big = [tb.SlicedBranch(tb.MISProblem(g_big), code, 0)]
+        assert tb.slice_footprint(too_big[0]) > (64 << 20)
+        g = tb.random_regular_graph(40, 3, seed=9)
+        ok = _sliced(tb, g, 6, seed=9)
+        with tb.Server(be) as srv:
+            t_ok0 = srv.submit(ok)
+            assert srv.wait(t_ok0) == orc.mis_exact(g)
+            t_bad = srv.submit(too_big)
+            with pytest.raises(tb.TbhError):
+                srv.wait(t_bad)
+            t_ok = srv.submit(ok)
+            assert srv.wait(t_ok) == orc.mis_exact(g)
+    finally:
+        be.close()
+
+
+def test_submit_errors(tb, be):
+    from tensorbranching_jl_amd import _lib
+    with tb.Server(be) as srv:
+        with pytest.raises(tb.TbhError):
+            srv.submit([])
+        with pytest.raises(tb.TbhError):
+            srv.poll(12345)
+    # after stop the serving entry points refuse
+    with pytest.raises(tb.TbhError):
+        _lib.check(_lib.tbh().tbh_cancel(be.handle, 1))

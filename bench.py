@@ -55,7 +55,7 @@ def cpu_baseline(br, gpu_results, seconds, threads):
     tcs = np.array([tb.complexity(b).tc if b.code is not None else -np.inf for b in br])
     work = np.where(np.isfinite(tcs), 2.0 ** tcs, 0.0)
     # slices small enough that a chunk stays within the time budget
-    order = [k for k in rng.permutation(len(br)) if tcs[k] <= 30]
+    order = [k for k in rng.permutation(len(br)) if tcs[k] <= 33]
     order = np.array(order, dtype=np.int64)
     sample, t_used = [], 0.0
     # grow the sample until about `seconds` of CPU time has been spent
@@ -79,14 +79,187 @@ def cpu_baseline(br, gpu_results, seconds, threads):
         "cores": threads,
         "kind": "port",
         "sample": (f"{len(sample)} of {len(br)} slices drawn at random (seed 0) among those "
-                   f"with tc <= 30, "
+                   f"with tc <= 33, "
                    f"{t_total:.1f}s on {threads} threads, scaled to the full solve by "
                    f"algorithmic work 2^tc (sample = {100 * frac:.2f}% of it); "
                    "results checked equal to the GPU's"),
     }
 
 
+def _pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, max(0, math.ceil(q * len(xs)) - 1))]
+
+
+def _drain(srv, tickets):
+    """Polls every ticket to completion: {ticket: (value, t_submit, t_done)}."""
+    out = {}
+    pending = list(tickets)
+    while pending:
+        rest = []
+        for t in pending:
+            st, v, t0, t1 = srv.poll(t)
+            if st == "done":
+                out[t] = (v, t0, t1)
+            elif st == "failed":
+                raise RuntimeError(f"served request {t} failed")
+            else:
+                rest.append(t)
+        pending = rest
+        if pending:
+            time.sleep(0.0005)
+    return out
+
+
+def serving_leg(tb, be, br, mis, copies, mix_count, seed):
+    """The continuous-batching scheduler: (a) `copies` requests of the main
+    workload submitted at once (served throughput); (b) a mixed request
+    stream (serving_requests) first as a burst (capacity), then as Poisson
+    arrivals at 70% of that capacity (latency p50 / p99 from the
+    scheduler's submit and completion stamps).  Every value is checked
+    against the synchronous batched call."""
+    t = time.time()
+    mix = tb.serving_requests(mix_count, seed=seed)
+    want = [float(be.contract_max(b, np.float32, "batched")) for _, _, b in mix]
+    log(f"serving mix: {mix_count} requests, "
+        f"{sum(len(b) for _, _, b in mix)} slices, generated in {time.time() - t:.1f}s")
+    out = {}
+    with tb.Server(be) as srv:
+        # (a) main workload, burst
+        tick = [srv.submit(br) for _ in range(copies)]
+        res = _drain(srv, tick)
+        t0 = min(r[1] for r in res.values())
+        t1 = max(r[2] for r in res.values())
+        if any(float(r[0]) != float(mis) for r in res.values()):
+            raise RuntimeError("served MIS differs from the batched call")
+        out["main_burst"] = {"requests": copies, "solves_per_s": round(copies / (t1 - t0), 4),
+                             "window_s": round(t1 - t0, 3)}
+        # (b) mixed stream: burst for capacity
+        tick = [srv.submit(b) for _, _, b in mix]
+        res = _drain(srv, tick)
+        for t, w in zip(tick, want):
+            if float(res[t][0]) != w:
+                raise RuntimeError("served mixed-stream value differs from the batched call")
+        t0 = min(r[1] for r in res.values())
+        t1 = max(r[2] for r in res.values())
+        cap = len(mix) / (t1 - t0)
+        # Poisson arrivals at 70% of capacity, the stream twice over
+        rate = 0.7 * cap
+        rng = np.random.default_rng(seed + 1)
+        order = list(range(len(mix))) * 2
+        gaps = rng.exponential(1.0 / rate, len(order))
+        tick = []
+        start = time.perf_counter()
+        due = 0.0
+        for k, gap in zip(order, gaps):
+            due += gap
+            dt = start + due - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tick.append((srv.submit(mix[k][2]), k))
+        res = _drain(srv, [t for t, _ in tick])
+        lat = []
+        for t, k in tick:
+            if float(res[t][0]) != want[k]:
+                raise RuntimeError("served Poisson-stream value differs from the batched call")
+            lat.append(res[t][2] - res[t][1])
+        t0 = min(r[1] for r in res.values())
+        t1 = max(r[2] for r in res.values())
+        out["mixed_stream"] = {
+            "requests": len(mix), "slices": sum(len(b) for _, _, b in mix),
+            "burst_capacity_req_per_s": round(cap, 3),
+            "poisson_rate_req_per_s": round(rate, 3), "arrivals": len(order),
+            "served_req_per_s": round(len(order) / (t1 - t0), 3),
+            "latency_ms": {"p50": round(1e3 * _pct(lat, 0.5), 2),
+                           "p99": round(1e3 * _pct(lat, 0.99), 2),
+                           "max": round(1e3 * max(lat), 2)},
+            "mix": "serving_requests(%d, seed=%d): 3-regular even n ~ U[100,260] and ER n ~ "
+                   "U[100,250] p=3/(n-1), sc_target ~ U[max(15, n//10-3), 25]" % (mix_count, seed),
+        }
+    return out
+
+
+def calibrate_copy():
+    """A plain copy-like kernel of known size for scaling the FETCH_SIZE /
+    WRITE_SIZE counters: y = -x over 2^28 floats (reads 1 GiB, writes 1 GiB),
+    three times.  Runs in its own process (torch only)."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "torch.neg", "read_bytes": 4 * n, "write_bytes": 4 * n,
+                      "reps": 3}))
+
+
+def _pmc_rows(d):
+    import csv
+    import glob
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            rows.extend(csv.DictReader(fh))
+    if not rows:
+        raise RuntimeError(f"no counter_collection.csv under {d}")
+    return rows
+
+
+def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out_json):
+    """Per-kernel HBM bytes per dispatch from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE: KiB per dispatch), each divided by the factor
+    the same counter shows on the calibration copy (measured / true bytes)."""
+    factors = {}
+    for name, d, true_key in (("FETCH_SIZE", copy_fetch_dir, "read_bytes"),
+                              ("WRITE_SIZE", copy_write_dir, "write_bytes")):
+        v = [float(r["Counter_Value"]) for r in _pmc_rows(d)
+             if r["Counter_Name"] == name and "neg" in r["Kernel_Name"]]
+        if not v:
+            raise RuntimeError(f"no calibration dispatch with {name} in {d}")
+        factors[name] = (sum(v) / len(v)) * 1024 / (4 << 28)
+    per = {}
+    for name, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        for r in _pmc_rows(d):
+            if r["Counter_Name"] != name:
+                continue
+            k = r["Kernel_Name"].split("(")[0].replace("void ", "")
+            e = per.setdefault(k, {})
+            e.setdefault(name, []).append(float(r["Counter_Value"]) * 1024 / factors[name])
+    kernels = {}
+    for k, e in per.items():
+        f, w = e.get("FETCH_SIZE", []), e.get("WRITE_SIZE", [])
+        if not f or not w:
+            continue
+        kernels[k] = {"dispatches": len(f),
+                      "fetch_bytes_per_dispatch": sum(f) / len(f),
+                      "write_bytes_per_dispatch": sum(w) / len(w),
+                      "traffic_bytes_per_dispatch": sum(f) / len(f) + sum(w) / len(w)}
+    res = {"factors": factors, "kernels": kernels}
+    with open(out_json, "w") as fh:
+        json.dump(res, fh, indent=1, sort_keys=True)
+    print(json.dumps(res))
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c3_3regular_n300_sc25.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` on this workload from the committed
+    PMC summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh)["kernels"][kernel]["traffic_bytes_per_dispatch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
+    if "--calibrate-copy" in sys.argv:
+        return calibrate_copy()
+    if "--summarize-pmc" in sys.argv:
+        i = sys.argv.index("--summarize-pmc")
+        return summarize_pmc(*sys.argv[i + 1:i + 6])
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -100,6 +273,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-copies", type=int, default=3)
+    ap.add_argument("--serve-mix", type=int, default=24)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched steps (for profiler runs)")
     args = ap.parse_args()
@@ -168,10 +344,11 @@ def main():
     dom = max(range(2), key=lambda k: sp["kind_ms"][k])  # EW kernels are HBM-bound
     dom_name = kinds[dom]
     achieved = sp["kind_bytes"][dom] / sp["kind_ms"][dom] / 1e6
+    kname = f"tbh::tbh_ew_{'direct_' if dom == 0 else ''}kernel<float>"
     roofline = {
-        "bound": "hbm", "kernel": f"tbh_ew_{'direct_' if dom == 0 else ''}kernel<float>",
+        "bound": "hbm", "kernel": kname,
         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname),
         "per_launch_algo_bytes": sp["kind_bytes"][dom] / max(1, sp["kind_launches"][dom]),
         "avg_launch_ms": sp["kind_ms"][dom] / max(1, sp["kind_launches"][dom]),
     }
@@ -197,6 +374,11 @@ def main():
         gemm = {"M": n, "N": n, "K": n, "dtype": "f32", "kernel_ms": round(ms, 3),
                 "TOPs": round(tops, 2), "valu_maxplus_ceiling_TOPs": round(VALU_MAXPLUS_TOPS, 1),
                 "frac_of_ceiling": round(tops / VALU_MAXPLUS_TOPS, 3)}
+
+    serving = None
+    if not args.no_serve and not args.batched_only:
+        serving = serving_leg(tb, be, br, mis, args.serve_copies, args.serve_mix, seed=7)
+        log(f"serving: {serving}")
 
     cpu = None
     if not args.no_cpu and not args.batched_only:
@@ -224,11 +406,11 @@ def main():
             "slices": len(br), "mis": float(mis), "pool_gb": args.pool_gb,
             "parallelism": "1 GPU",
         },
-        "latency_ms": {"p50": round(lat_ms[len(lat_ms) // 2], 2),
-                       "p99": round(lat_ms[min(len(lat_ms) - 1, math.ceil(0.99 * len(lat_ms)) - 1)], 2)},
+        "latency_ms": {"p50": round(_pct(lat_ms, 0.5), 2), "p99": round(_pct(lat_ms, 0.99), 2)},
         "plain_gpu_loop": None if t_plain is None else {
             "solves_per_s": round(1.0 / t_plain, 4), "ms_per_solve": round(1e3 * t_plain, 1),
             "batched_speedup": round(solves_per_s * t_plain, 2)},
+        "serving": serving,
         "tropical_GOPs": round(st_timed["total_algo_ops"] / t_total / 1e9, 1),
         "roofline": roofline,
         "kernels": per_kind,

@@ -17,5 +17,5 @@ from .api import (  # noqa: F401
     solve_slice, tc)
 from .workload import (  # noqa: F401
     ContractionTreeSlicer, dynamic_ob_mis, erdos_renyi, initialize_code, random_ksg,
-    random_regular_graph, slice_graph)
+    random_regular_graph, serving_requests, slice_graph)
 from .serve import Server  # noqa: F401

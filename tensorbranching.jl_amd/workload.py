@@ -142,3 +142,28 @@ def dynamic_ob_mis(g, weights=None, *, slicer=None, usecuda=True, element_type=n
     branches = slice_graph(g, weights, slicer, seed=seed)
     res = contract_slices(branches, element_type, usecuda, mode=mode, backend=backend)
     return res.max()
+
+
+def serving_requests(count, seed=0, order_trials=50):
+    """A mixed stream of MIS requests for the serving scheduler: random
+    3-regular graphs with even n ~ U[100, 260] and Erdos-Renyi graphs with n ~
+    U[100, 250], p = 3 / (n - 1), each sliced to sc_target ~ U[lo(n), 25]
+    with lo(n) = max(15, n // 10 - 3) (lower targets multiply the slice count
+    of large graphs into the 10^5 range).  Returns a list of
+    (description, graph, branches)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        gseed = int(rng.integers(1 << 31))
+        if rng.random() < 0.5:
+            n = 2 * int(rng.integers(50, 131))  # n * 3 must be even
+            g, kind = random_regular_graph(n, 3, seed=gseed), "3-regular"
+        else:
+            n = int(rng.integers(100, 251))
+            g, kind = erdos_renyi(n, 3.0 / (n - 1), seed=gseed), "ER"
+        lo = max(15, n // 10 - 3)
+        sc_target = int(rng.integers(lo, 26))
+        br = slice_graph(g, None, ContractionTreeSlicer(sc_target=sc_target,
+                                                        order_trials=order_trials), seed=gseed)
+        out.append((f"{kind} n={n} seed={gseed} sc_target={sc_target}", g, br))
+    return out

@@ -160,12 +160,23 @@ typedef struct {
   int32_t n_reduced;        /* EW: reduced labels                           */
   int32_t M, N, K, batch;   /* GEMM extents                                 */
   int32_t lanes_a, lanes_b; /* EW: contiguous A/B positions under lane bits */
+  int32_t tile_bits;        /* EW: outputs per tile (log2)                  */
+  int32_t outer_reduced;    /* EW: reduced labels looped outside LDS        */
+  int32_t direct;           /* EW: 1 = direct (wave) step, 0 = LDS-tiled    */
+  int32_t pad_;
   double macs, bytes;       /* executed configurations, algorithmic bytes   */
 } tbh_step_info;
 /* Plans one slice and writes up to cap step records; returns the number of
  * steps (>= 0) or a negative error code. */
 int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode,
                    tbh_step_info* out, int cap);
+
+/* Measured VALU ceiling of the max-plus instruction mix (no reference
+ * counterpart: the denominator of the GEMM's roofline).  Runs a
+ * register-only kernel with the GEMM inner loop's mix -- 2 adds and 1 max3
+ * per 2 tropical MACs, 8 x 8 register blocks -- on every CU and returns
+ * tropical ops/s (2 ops per MAC). */
+int tbh_valu_peak(tbh_ctx* ctx, tbh_dtype dtype, double* tropical_ops_per_s);
 
 typedef struct {
   uint64_t launches;        /* kernel launches issued                     */

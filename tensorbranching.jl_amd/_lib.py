@@ -43,8 +43,9 @@ class TbhStepInfo(C.Structure):
     _fields_ = [("kind", C.c_int32), ("level", C.c_int32), ("rank_a", C.c_int32),
                 ("rank_b", C.c_int32), ("rank_c", C.c_int32), ("n_reduced", C.c_int32),
                 ("M", C.c_int32), ("N", C.c_int32), ("K", C.c_int32), ("batch", C.c_int32),
-                ("lanes_a", C.c_int32), ("lanes_b", C.c_int32), ("macs", C.c_double),
-                ("bytes", C.c_double)]
+                ("lanes_a", C.c_int32), ("lanes_b", C.c_int32), ("tile_bits", C.c_int32),
+                ("outer_reduced", C.c_int32), ("direct", C.c_int32), ("pad_", C.c_int32),
+                ("macs", C.c_double), ("bytes", C.c_double)]
 
 
 class TbwSliceView(C.Structure):
@@ -73,6 +74,7 @@ TBH_SIGNATURES = {
     "tbh_peak_memory": (C.c_int, [C.POINTER(TbhSlice), C.POINTER(C.c_double)]),
     "tbh_all_memory": (C.c_int, [C.POINTER(TbhSlice), C.POINTER(C.c_double)]),
     "tbh_slice_footprint": (C.c_int, [C.POINTER(TbhSlice), C.c_int, C.POINTER(C.c_size_t)]),
+    "tbh_valu_peak": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "tbh_serve_start": (C.c_int, [C.c_void_p, C.c_size_t]),
     "tbh_serve_stop": (C.c_int, [C.c_void_p]),
     "tbh_submit": (C.c_int, [C.c_void_p, C.POINTER(TbhRequest), C.POINTER(C.c_uint64)]),

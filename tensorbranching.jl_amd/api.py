@@ -275,6 +275,14 @@ class HIPBackend:
                                           _mode(mode)))
         return out[0]
 
+    def valu_peak(self, element_type=np.float32):
+        """Measured tropical ops/s of the GEMM's VALU instruction mix with
+        register operands (the roofline ceiling of compute-bound steps)."""
+        code, _ = _dtype_code(element_type)
+        v = C.c_double()
+        check(_lib.tbh().tbh_valu_peak(self.handle, code, C.byref(v)))
+        return v.value
+
     def maxplus_gemm(self, A, B, C_out=None):
         """C = A (x) B in the tropical semiring; A (M,K[,batch]) and B
         (K,N[,batch]) are column-major (Fortran-ordered) numpy arrays, or

@@ -22,13 +22,19 @@ constexpr int kLdsElems = (1 << kSubBits) + ((1 << kSubBits) >> 5);  // padded
 // nSB bits in ascending operand position, so consecutive threads read
 // consecutive addresses along the operand's lowest positions -- into LDS,
 // then each of its outputs maxes A + B over the 2^nR reduced configurations
-// and is written back contiguously along C's lowest positions.
+// and is written back contiguously along C's lowest positions.  Reduced
+// labels beyond the sub-tensor budget are outer reduced labels (nRo): the
+// tile repeats the load + max for each of their 2^nRo configurations,
+// keeping its maxima in registers, before it writes.
 // All positions are label positions (stride 2^p); -1 marks an absent label.
 struct EwDesc {
   uint64_t a, b, c;        // device addresses; b == 0: unary step (B = tropical 1)
-  int32_t nT, nO, nR;      // tile, outer and reduced bits
+  int32_t nT, nO, nR;      // tile, outer and (sub-tensor) reduced bits
   int32_t nSA, nSB;        // sub-tensor bits of A and B
   int32_t tiles;           // 2^nO
+  int32_t nRo;             // outer reduced bits (tiled steps only)
+  int8_t qA[kMaxReduce];   // outer reduced bit -> A / B position
+  int8_t qB[kMaxReduce];
   int8_t tC[kTileBits];    // tile bit -> C position
   int8_t tA[kTileBits];    // tile bit -> bit of A's sub-tensor
   int8_t tB[kTileBits];

@@ -12,6 +12,7 @@
 //   K3 tbh_leaf_kernel    leaf tensors of the MIS network from weights.
 //   K5 tbh_epi_kernel     per-slice root + r.
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "tbh_internal.hpp"
@@ -23,8 +24,39 @@
 #ifndef TBH_DIRECT_WAVES
 #define TBH_DIRECT_WAVES 4
 #endif
+#ifndef TBH_GEMM_WAVES
+#define TBH_GEMM_WAVES 2
+#endif
 
 namespace tbh {
+
+// Address space of the step descriptors and tile prefixes (uploaded before a
+// launch, never written by a kernel).  Generic by default: their fields are
+// then read with vector loads.  Reading them through the constant address
+// space (-DTBH_KCONST='__attribute__((address_space(4)))') turns the reads
+// into scalar loads, which measured 3x slower for the direct kernel, whose
+// waves re-read a descriptor (and search the prefixes) every few tiles.
+#ifndef TBH_KCONST
+#define TBH_KCONST
+#endif
+using KEw = const TBH_KCONST EwDesc*;
+// An int8 position array of a descriptor, read through its containing
+// dwords (one load per four positions when the compiler merges them).
+struct KI8 {
+  KEw d;
+  uint32_t off;
+  __device__ __forceinline__ int operator[](uint32_t i) const {
+    const uint32_t o = off + i;
+    const uint32_t w = reinterpret_cast<const TBH_KCONST uint32_t*>(d)[o >> 2];
+    return (int)(int8_t)(w >> ((o & 3) * 8));
+  }
+};
+#define TBH_KF(d, f) (KI8{(d), (uint32_t)offsetof(EwDesc, f)})
+// Tensors are addressed in the global address space: their loads are
+// global_load (counted by vmcnt only), not flat loads, which also count in
+// lgkmcnt and so would make every wait for an LDS or scalar access wait for
+// the data loads in flight too.
+#define TBH_GLOBAL __attribute__((address_space(1)))
 
 template <typename T>
 struct Trop;
@@ -49,6 +81,19 @@ struct Trop<int32_t> {
     return x < kI32NegThresh ? kI32NegInf : x;
   }
 };
+// max(a, b, c) as one v_max3 (the compiler forms it for only part of the
+// GEMM's reductions).  Register operands only; inputs are never NaN.
+__device__ __forceinline__ float max3(float a, float b, float c) {
+  float r;
+  asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+__device__ __forceinline__ int32_t max3(int32_t a, int32_t b, int32_t c) {
+  int32_t r;
+  asm("v_max3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 // zero used to pad GEMM operands (adds without overflow for Int32)
 template <typename T>
 __device__ __forceinline__ T pad_zero();
@@ -60,7 +105,7 @@ __device__ __forceinline__ int32_t pad_zero<int32_t>() { return kI32NegInf; }
 // ---------------------------------------------------------------- K2 -----
 // The 4 high bits (8..11) of a per-thread index: strides of those bits, and
 // the sum over the bits set in a compile-time m.
-__device__ __forceinline__ void hi_strides(uint32_t (&h)[4], const int8_t* pos) {
+__device__ __forceinline__ void hi_strides(uint32_t (&h)[4], KI8 pos) {
 #pragma unroll
   for (int b = 0; b < 4; b++) h[b] = pos_stride(pos[8 + b]);
 }
@@ -75,7 +120,7 @@ __device__ __forceinline__ uint32_t hi_at(const uint32_t (&h)[4], int m) {
 }
 
 // the low 8 bits of the thread index mapped through pos[0..7]
-__device__ __forceinline__ uint32_t lo_dep(uint32_t tid, const int8_t* pos) {
+__device__ __forceinline__ uint32_t lo_dep(uint32_t tid, KI8 pos) {
   uint32_t v = 0;
 #pragma unroll
   for (int b = 0; b < 8; b++)
@@ -88,43 +133,52 @@ struct EwThread {  // per-step constants of one thread
   uint32_t hSA[4], hSB[4], hTA[4], hTB[4], hTC[4];
 };
 
-__device__ __forceinline__ void ew_thread_setup(const EwDesc* __restrict__ d, EwThread& e) {
+__device__ __forceinline__ void ew_thread_setup(KEw d, EwThread& e) {
   const uint32_t tid = threadIdx.x;
-  e.loSA = lo_dep(tid, d->sA);
-  e.loSB = lo_dep(tid, d->sB);
-  e.loTA = lo_dep(tid, d->tA);
-  e.loTB = lo_dep(tid, d->tB);
-  e.loTC = lo_dep(tid, d->tC);
-  hi_strides(e.hSA, d->sA);
-  hi_strides(e.hSB, d->sB);
-  hi_strides(e.hTA, d->tA);
-  hi_strides(e.hTB, d->tB);
-  hi_strides(e.hTC, d->tC);
+  e.loSA = lo_dep(tid, TBH_KF(d, sA));
+  e.loSB = lo_dep(tid, TBH_KF(d, sB));
+  e.loTA = lo_dep(tid, TBH_KF(d, tA));
+  e.loTB = lo_dep(tid, TBH_KF(d, tB));
+  e.loTC = lo_dep(tid, TBH_KF(d, tC));
+  hi_strides(e.hSA, TBH_KF(d, sA));
+  hi_strides(e.hSB, TBH_KF(d, sB));
+  hi_strides(e.hTA, TBH_KF(d, tA));
+  hi_strides(e.hTB, TBH_KF(d, tB));
+  hi_strides(e.hTC, TBH_KF(d, tC));
+}
+
+// The step index and its tile range are the same in every lane of a wave;
+// saying so keeps them in scalar registers and the step loop's control flow
+// uniform.
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 // outer offsets of tile o (sum of the strides of its set bits)
 struct EwOuter {
   uint32_t a, b, c;
 };
-__device__ __forceinline__ EwOuter outer_offsets(const EwDesc* __restrict__ d, uint32_t o) {
+__device__ __forceinline__ EwOuter outer_offsets(KEw d, uint32_t o) {
   EwOuter r{0, 0, 0};
   const int nO = d->nO;
 #pragma unroll
   for (int k = 0; k < 32; k++)
     if (k < nO && ((o >> k) & 1)) {
-      r.a += pos_stride(d->oA[k]);
-      r.b += pos_stride(d->oB[k]);
-      r.c += pos_stride(d->oC[k]);
+      r.a += pos_stride(TBH_KF(d, oA)[k]);
+      r.b += pos_stride(TBH_KF(d, oB)[k]);
+      r.c += pos_stride(TBH_KF(d, oC)[k]);
     }
   return r;
 }
 // from tile o-1 to tile o: only the bits that flip change
-__device__ __forceinline__ void outer_step(const EwDesc* __restrict__ d, uint32_t o, EwOuter& r) {
+__device__ __forceinline__ void outer_step(KEw d, uint32_t o, EwOuter& r) {
   uint32_t flip = o ^ (o - 1);
   while (flip) {
     const int k = __builtin_ctz(flip);
     flip &= flip - 1;
-    const uint32_t sa = pos_stride(d->oA[k]), sb = pos_stride(d->oB[k]), sc = pos_stride(d->oC[k]);
+    const uint32_t sa = pos_stride(TBH_KF(d, oA)[k]), sb = pos_stride(TBH_KF(d, oB)[k]), sc = pos_stride(TBH_KF(d, oC)[k]);
     if ((o >> k) & 1) {
       r.a += sa;
       r.b += sb;
@@ -137,125 +191,164 @@ __device__ __forceinline__ void outer_step(const EwDesc* __restrict__ d, uint32_
   }
 }
 
-// Reduce phase of a tile with CNT = max(1, 2^nT / 256) outputs per thread.
-// A sub-tensor index is a sum of distinct powers of two (tile bits and
-// reduced bits sit at different positions), and lds_pad(x + y) =
-// lds_pad(x) + lds_pad(y) when x and y share no bit, so the padded address of
-// (output m, reduced configuration) is a per-thread part plus a wave-uniform
-// part: one add per LDS read.
-template <typename T, bool HASB, int CNT>
-__device__ __forceinline__ void ew_tile_reduce(const EwDesc* __restrict__ d, const EwThread& e,
-                                               const T* __restrict__ lA,
-                                               const T* __restrict__ lB, T* __restrict__ C) {
-  const int tid = threadIdx.x;
-  if (CNT == 1 && tid >= (1 << d->nT)) return;
-  uint32_t pA[CNT], pB[CNT];
-  T acc[CNT];
-#pragma unroll
-  for (int m = 0; m < CNT; m++) {
-    acc[m] = Trop<T>::zero();
-    pA[m] = lds_pad(e.loTA + hi_at(e.hTA, m));
-    if (HASB) pB[m] = lds_pad(e.loTB + hi_at(e.hTB, m));
-  }
-  // the lowest (up to 3) reduced bits unrolled in pairs (add + max3), the
-  // rest walked in Gray-code order, one stride change per step
-  const int nR = d->nR;
-  const int nlo = nR < 3 ? nR : 3;
-  uint32_t lsA[4] = {0, 0, 0, 0}, lsB[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int b = 0; b < 3; b++)
-    if (b < nlo) {
-      lsA[b] = pos_stride(d->rA[b]);
-      lsB[b] = pos_stride(d->rB[b]);
-    }
-  const int nq = 1 << nlo;
-  const uint32_t nh = 1u << (nR - nlo);
-  uint32_t rA = 0, rB = 0;
-  for (uint32_t h = 0; h < nh; h++) {
-    if (h) {
-      const int i = __builtin_ctz(h);
-      const uint32_t g = h ^ (h >> 1);
-      const uint32_t sa = pos_stride(d->rA[nlo + i]), sb = pos_stride(d->rB[nlo + i]);
-      if ((g >> i) & 1) {
-        rA += sa;
-        rB += sb;
-      } else {
-        rA -= sa;
-        rB -= sb;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 8; q += 2) {
-      if (q >= nq) break;
-      const uint32_t qa0 = lds_pad(rA + hi_at(lsA, q)), qb0 = lds_pad(rB + hi_at(lsB, q));
-      if (nq > 1) {
-        const uint32_t qa1 = lds_pad(rA + hi_at(lsA, q + 1));
-        const uint32_t qb1 = lds_pad(rB + hi_at(lsB, q + 1));
-#pragma unroll
-        for (int m = 0; m < CNT; m++) {
-          T x0 = lA[pA[m] + qa0], x1 = lA[pA[m] + qa1];
-          if (HASB) {
-            x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
-            x1 = Trop<T>::mul(x1, lB[pB[m] + qb1]);
-          }
-          acc[m] = Trop<T>::add(acc[m], Trop<T>::add(x0, x1));
-        }
-      } else {
-#pragma unroll
-        for (int m = 0; m < CNT; m++) {
-          T x0 = lA[pA[m] + qa0];
-          if (HASB) x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
-          acc[m] = Trop<T>::add(acc[m], x0);
-        }
-      }
-    }
-  }
-  // contiguous writes along C's lowest positions
-#pragma unroll
-  for (int m = 0; m < CNT; m++) C[e.loTC + hi_at(e.hTC, m)] = Trop<T>::canon(acc[m]);
-}
-
 // the sub-tensor of one operand (2^nS elements) into LDS; lds_pad(tid + 256m)
 // = lds_pad(tid) + 264m
 template <typename T>
-__device__ __forceinline__ void ew_stage(const T* __restrict__ X, int nS, uint32_t lo,
+__device__ __forceinline__ void ew_stage(const TBH_GLOBAL T* __restrict__ X, int nS, uint32_t lo,
                                          const uint32_t (&hS)[4], T* __restrict__ lX) {
   const int tid = threadIdx.x;
   const int n = 1 << nS;
+  // recomputed per call: addresses are not held across the caller's loop
+  asm volatile("" : "+v"(lo));
   const uint32_t pt = lds_pad(tid);
   if (n <= 256) {
     if (tid < n) lX[pt] = X[lo];
     return;
   }
   const int cnt = n >> 8;  // 2..16, wave-uniform
-  T v[16];
+  // in groups of 8 loads in flight (few registers: the tile's maxima stay
+  // live across the staging)
 #pragma unroll
-  for (int m = 0; m < 16; m++)
-    if (m < cnt) v[m] = X[lo + hi_at(hS, m)];
+  for (int g = 0; g < 16; g += 8) {
+    if (g >= cnt) break;
+    T v[8];
 #pragma unroll
-  for (int m = 0; m < 16; m++)
-    if (m < cnt) lX[pt + 264 * m] = v[m];
+    for (int m = 0; m < 8; m++)
+      if (g + m < cnt) v[m] = X[lo + hi_at(hS, g + m)];
+#pragma unroll
+    for (int m = 0; m < 8; m++)
+      if (g + m < cnt) lX[pt + 264 * (g + m)] = v[m];
+  }
 }
 
-// One tile of step d at outer offsets off.
-template <typename T, bool HASB>
-__device__ __forceinline__ void ew_tile(const EwDesc* __restrict__ d, const EwOuter& off,
+// One tile of step d at outer offsets off, CNT = max(1, 2^nT / 256) outputs
+// per thread.  For each configuration of the outer reduced labels (Gray-code
+// order, one stride change each) the tile stages its sub-tensors of A and B
+// in LDS and maxes A + B over the sub-tensor reduced labels into registers;
+// the outputs are written once at the end.
+// A sub-tensor index is a sum of distinct powers of two (tile bits and
+// reduced bits sit at different positions), and lds_pad(x + y) =
+// lds_pad(x) + lds_pad(y) when x and y share no bit, so the padded address of
+// (output m, reduced configuration) is a per-thread part plus a wave-uniform
+// part: one add per LDS read.
+template <typename T, bool HASB, int CNT>
+__device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
                                         const EwThread& e, T* __restrict__ lA,
                                         T* __restrict__ lB) {
-  const T* __restrict__ A = reinterpret_cast<const T*>(d->a) + off.a;
-  const T* __restrict__ B = reinterpret_cast<const T*>(d->b) + off.b;
-  T* __restrict__ C = reinterpret_cast<T*>(d->c) + off.c;
-  ew_stage<T>(A, d->nSA, e.loSA, e.hSA, lA);
-  if (HASB) ew_stage<T>(B, d->nSB, e.loSB, e.hSB, lB);
-  __syncthreads();
-  switch (d->nT <= 8 ? 0 : d->nT - 8) {
-    case 0: ew_tile_reduce<T, HASB, 1>(d, e, lA, lB, C); break;
-    case 1: ew_tile_reduce<T, HASB, 2>(d, e, lA, lB, C); break;
-    case 2: ew_tile_reduce<T, HASB, 4>(d, e, lA, lB, C); break;
-    case 3: ew_tile_reduce<T, HASB, 8>(d, e, lA, lB, C); break;
-    default: ew_tile_reduce<T, HASB, 16>(d, e, lA, lB, C); break;
+  const int tid = threadIdx.x;
+  const TBH_GLOBAL T* __restrict__ A = reinterpret_cast<const TBH_GLOBAL T*>(d->a) + off.a;
+  const TBH_GLOBAL T* __restrict__ B = reinterpret_cast<const TBH_GLOBAL T*>(d->b) + off.b;
+  TBH_GLOBAL T* __restrict__ C = reinterpret_cast<TBH_GLOBAL T*>(d->c) + off.c;
+  const bool active = CNT > 1 || tid < (1 << d->nT);
+  T acc[CNT];
+#pragma unroll
+  for (int m = 0; m < CNT; m++) acc[m] = Trop<T>::zero();
+  // the lowest (up to 3) sub-tensor reduced bits unrolled in pairs (add +
+  // max3), the rest walked in Gray-code order
+  const int nR = d->nR;
+  const int nlo = nR < 3 ? nR : 3;
+  uint32_t lsA[4] = {0, 0, 0, 0}, lsB[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int b = 0; b < 3; b++)
+    if (b < nlo) {
+      lsA[b] = pos_stride(TBH_KF(d, rA)[b]);
+      lsB[b] = pos_stride(TBH_KF(d, rB)[b]);
+    }
+  const int nq = 1 << nlo;
+  const uint32_t nh = 1u << (nR - nlo);
+  const uint32_t nc = 1u << d->nRo;
+  uint32_t cA = 0, cB = 0;
+  for (uint32_t c = 0; c < nc; c++) {
+    if (c) {
+      const int i = __builtin_ctz(c);
+      const uint32_t g = c ^ (c >> 1);
+      const uint32_t sa = pos_stride(TBH_KF(d, qA)[i]), sb = pos_stride(TBH_KF(d, qB)[i]);
+      if ((g >> i) & 1) {
+        cA += sa;
+        cB += sb;
+      } else {
+        cA -= sa;
+        cB -= sb;
+      }
+    }
+    ew_stage<T>(A + cA, d->nSA, e.loSA, e.hSA, lA);
+    if (HASB) ew_stage<T>(B + cB, d->nSB, e.loSB, e.hSB, lB);
+    __syncthreads();
+    if (active) {
+      // per-thread LDS addresses, recomputed per chunk: the empty asm hides
+      // that they are loop-invariant, so they are not held in registers
+      // across the staging
+      uint32_t ltA = e.loTA, ltB = e.loTB;
+      asm volatile("" : "+v"(ltA), "+v"(ltB));
+      uint32_t pA[CNT], pB[CNT];
+#pragma unroll
+      for (int m = 0; m < CNT; m++) {
+        pA[m] = lds_pad(ltA + hi_at(e.hTA, m));
+        if (HASB) pB[m] = lds_pad(ltB + hi_at(e.hTB, m));
+      }
+      uint32_t rA = 0, rB = 0;
+      for (uint32_t h = 0; h < nh; h++) {
+        if (h) {
+          const int i = __builtin_ctz(h);
+          const uint32_t g = h ^ (h >> 1);
+          const uint32_t sa = pos_stride(TBH_KF(d, rA)[nlo + i]), sb = pos_stride(TBH_KF(d, rB)[nlo + i]);
+          if ((g >> i) & 1) {
+            rA += sa;
+            rB += sb;
+          } else {
+            rA -= sa;
+            rB -= sb;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+          if (q >= nq) break;
+          const uint32_t qa0 = lds_pad(rA + hi_at(lsA, q)), qb0 = lds_pad(rB + hi_at(lsB, q));
+          if (nq > 1) {
+            const uint32_t qa1 = lds_pad(rA + hi_at(lsA, q + 1));
+            const uint32_t qb1 = lds_pad(rB + hi_at(lsB, q + 1));
+#pragma unroll
+            for (int m = 0; m < CNT; m++) {
+              T x0 = lA[pA[m] + qa0], x1 = lA[pA[m] + qa1];
+              if (HASB) {
+                x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
+                x1 = Trop<T>::mul(x1, lB[pB[m] + qb1]);
+              }
+              acc[m] = Trop<T>::add(acc[m], Trop<T>::add(x0, x1));
+            }
+          } else {
+#pragma unroll
+            for (int m = 0; m < CNT; m++) {
+              T x0 = lA[pA[m] + qa0];
+              if (HASB) x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
+              acc[m] = Trop<T>::add(acc[m], x0);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  // contiguous writes along C's lowest positions
+  if (active) {
+    uint32_t lc = e.loTC;
+    asm volatile("" : "+v"(lc));
+#pragma unroll
+    for (int m = 0; m < CNT; m++) C[lc + hi_at(e.hTC, m)] = Trop<T>::canon(acc[m]);
+  }
+}
+
+template <typename T, bool HASB>
+__device__ __forceinline__ void ew_tile_dispatch(KEw d, const EwOuter& off,
+                                                 const EwThread& e, T* __restrict__ lA,
+                                                 T* __restrict__ lB) {
+  switch (d->nT <= 8 ? 0 : d->nT - 8) {
+    case 0: ew_tile<T, HASB, 1>(d, off, e, lA, lB); break;
+    case 1: ew_tile<T, HASB, 2>(d, off, e, lA, lB); break;
+    case 2: ew_tile<T, HASB, 4>(d, off, e, lA, lB); break;
+    case 3: ew_tile<T, HASB, 8>(d, off, e, lA, lB); break;
+    default: ew_tile<T, HASB, 16>(d, off, e, lA, lB); break;
+  }
 }
 
 // Direct steps: wave-level tiles of at most 2^10 outputs (lane = tile bits
@@ -265,7 +358,7 @@ __device__ __forceinline__ void ew_tile(const EwDesc* __restrict__ d, const EwOu
 // labels, so the loads are coalesced without LDS and without barriers.  The
 // operand position of tile bit k is sA[tA[k]] (sB[tB[k]]), of reduced bit i
 // sA[rA[i]], of outer bit k oA[k].
-__device__ __forceinline__ uint32_t sub_stride(const int8_t* s, int8_t t) {
+__device__ __forceinline__ uint32_t sub_stride(KI8 s, int t) {
   return t >= 0 ? pos_stride(s[t]) : 0u;
 }
 
@@ -274,32 +367,32 @@ struct EwLane {  // per-step constants of one lane
   uint32_t hA[4], hB[4], hC[4];
 };
 
-__device__ __forceinline__ void ew_lane_setup(const EwDesc* __restrict__ d, int lane, EwLane& L) {
+__device__ __forceinline__ void ew_lane_setup(KEw d, int lane, EwLane& L) {
   const int nT = d->nT;
   L.la = L.lb = L.lc = 0;
 #pragma unroll
   for (int k = 0; k < 6; k++)
     if (k < nT && ((lane >> k) & 1)) {
-      L.la += sub_stride(d->sA, d->tA[k]);
-      L.lb += sub_stride(d->sB, d->tB[k]);
-      L.lc += pos_stride(d->tC[k]);
+      L.la += sub_stride(TBH_KF(d, sA), TBH_KF(d, tA)[k]);
+      L.lb += sub_stride(TBH_KF(d, sB), TBH_KF(d, tB)[k]);
+      L.lc += pos_stride(TBH_KF(d, tC)[k]);
     }
 #pragma unroll
   for (int b = 0; b < 4; b++) {
-    L.hA[b] = 6 + b < nT ? sub_stride(d->sA, d->tA[6 + b]) : 0u;
-    L.hB[b] = 6 + b < nT ? sub_stride(d->sB, d->tB[6 + b]) : 0u;
-    L.hC[b] = 6 + b < nT ? pos_stride(d->tC[6 + b]) : 0u;
+    L.hA[b] = 6 + b < nT ? sub_stride(TBH_KF(d, sA), TBH_KF(d, tA)[6 + b]) : 0u;
+    L.hB[b] = 6 + b < nT ? sub_stride(TBH_KF(d, sB), TBH_KF(d, tB)[6 + b]) : 0u;
+    L.hC[b] = 6 + b < nT ? pos_stride(TBH_KF(d, tC)[6 + b]) : 0u;
   }
 }
 
 // CNT = max(1, 2^nT / 64) outputs per lane (nT < 6: the low 2^nT lanes).
 template <typename T, bool HASB, int CNT>
-__device__ __forceinline__ void ew_direct_tile(const EwDesc* __restrict__ d, const EwLane& L,
+__device__ __forceinline__ void ew_direct_tile(KEw d, const EwLane& L,
                                                const EwOuter& off, int lane) {
   if (CNT == 1 && lane >= (1 << d->nT)) return;
-  const T* __restrict__ A = reinterpret_cast<const T*>(d->a) + off.a;
-  const T* __restrict__ B = reinterpret_cast<const T*>(d->b) + off.b;
-  T* __restrict__ C = reinterpret_cast<T*>(d->c) + off.c;
+  const TBH_GLOBAL T* __restrict__ A = reinterpret_cast<const TBH_GLOBAL T*>(d->a) + off.a;
+  const TBH_GLOBAL T* __restrict__ B = reinterpret_cast<const TBH_GLOBAL T*>(d->b) + off.b;
+  TBH_GLOBAL T* __restrict__ C = reinterpret_cast<TBH_GLOBAL T*>(d->c) + off.c;
   const int nR = d->nR;
   uint32_t ia[CNT], ib[CNT];
   T acc[CNT];
@@ -311,8 +404,8 @@ __device__ __forceinline__ void ew_direct_tile(const EwDesc* __restrict__ d, con
   if (nR <= 1) {
     // the common spine step: both reduced configurations' loads in flight
     // before any is consumed
-    const uint32_t sA = nR ? sub_stride(d->sA, d->rA[0]) : 0u;
-    const uint32_t sB = nR ? sub_stride(d->sB, d->rB[0]) : 0u;
+    const uint32_t sA = nR ? sub_stride(TBH_KF(d, sA), TBH_KF(d, rA)[0]) : 0u;
+    const uint32_t sB = nR ? sub_stride(TBH_KF(d, sB), TBH_KF(d, rB)[0]) : 0u;
     T x0[CNT], x1[CNT];
 #pragma unroll
     for (int m = 0; m < CNT; m++) {
@@ -334,7 +427,7 @@ __device__ __forceinline__ void ew_direct_tile(const EwDesc* __restrict__ d, con
       if (r) {
         const int i = __builtin_ctz(r);
         const uint32_t g = r ^ (r >> 1);
-        const uint32_t sa = sub_stride(d->sA, d->rA[i]), sb = sub_stride(d->sB, d->rB[i]);
+        const uint32_t sa = sub_stride(TBH_KF(d, sA), TBH_KF(d, rA)[i]), sb = sub_stride(TBH_KF(d, sB), TBH_KF(d, rB)[i]);
         if ((g >> i) & 1) {
           rA += sa;
           rB += sb;
@@ -356,7 +449,7 @@ __device__ __forceinline__ void ew_direct_tile(const EwDesc* __restrict__ d, con
 }
 
 template <typename T, bool HASB>
-__device__ __forceinline__ void ew_direct_dispatch(const EwDesc* __restrict__ d, const EwLane& L,
+__device__ __forceinline__ void ew_direct_dispatch(KEw d, const EwLane& L,
                                                    const EwOuter& off, int lane) {
   switch (d->nT <= 6 ? 0 : d->nT - 6) {
     case 0: ew_direct_tile<T, HASB, 1>(d, L, off, lane); break;
@@ -392,21 +485,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBH_DIRECT_
         else
           hi = mid;
       }
-      op = lo;
-      ob = prefix[op];
-      oe = prefix[op + 1];
-      ew_lane_setup(ops + op, lane, L);
+      op = __builtin_amdgcn_readfirstlane(lo);
+      ob = uniform64(((const TBH_KCONST int64_t*)prefix)[op]);
+      oe = uniform64(((const TBH_KCONST int64_t*)prefix)[op + 1]);
+      ew_lane_setup((KEw)(ops + op), lane, L);
     }
     for (int64_t t = c0; t < c1; t++) {
       bool fresh = t == c0;
       while (t >= oe) {
         op++;
         ob = oe;
-        oe = prefix[op + 1];
+        oe = uniform64(((const TBH_KCONST int64_t*)prefix)[op + 1]);
         fresh = true;
-        if (t < oe) ew_lane_setup(ops + op, lane, L);
+        if (t < oe) ew_lane_setup((KEw)(ops + op), lane, L);
       }
-      const EwDesc* d = ops + op;
+      KEw d = (KEw)(ops + op);
       const uint32_t o = (uint32_t)(t - ob);
       if (fresh)
         off = outer_offsets(d, o);
@@ -443,10 +536,10 @@ __global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW
         else
           hi = mid;
       }
-      op = lo;
-      ob = prefix[op];
-      oe = prefix[op + 1];
-      ew_thread_setup(ops + op, e);
+      op = __builtin_amdgcn_readfirstlane(lo);
+      ob = uniform64(((const TBH_KCONST int64_t*)prefix)[op]);
+      oe = uniform64(((const TBH_KCONST int64_t*)prefix)[op + 1]);
+      ew_thread_setup((KEw)(ops + op), e);
     }
     EwOuter off{0, 0, 0};
     for (int64_t t = c0; t < c1; t++) {
@@ -454,20 +547,20 @@ __global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW
       while (t >= oe) {
         op++;
         ob = oe;
-        oe = prefix[op + 1];
+        oe = uniform64(((const TBH_KCONST int64_t*)prefix)[op + 1]);
         fresh = true;
-        if (t < oe) ew_thread_setup(ops + op, e);
+        if (t < oe) ew_thread_setup((KEw)(ops + op), e);
       }
-      const EwDesc* d = ops + op;
+      KEw d = (KEw)(ops + op);
       const uint32_t o = (uint32_t)(t - ob);
       if (fresh)
         off = outer_offsets(d, o);
       else
         outer_step(d, o, off);
       if (d->b != 0)
-        ew_tile<T, true>(d, off, e, lA, lB);
+        ew_tile_dispatch<T, true>(d, off, e, lA, lB);
       else
-        ew_tile<T, false>(d, off, e, lA, lB);
+        ew_tile_dispatch<T, false>(d, off, e, lA, lB);
     }
   }
 }
@@ -477,21 +570,23 @@ constexpr int BM = 128, BN = 128, BK = 16, BPAD = 4;
 
 template <typename T>
 struct Vec4;
+// clang vector types (they copy across address spaces, unlike HIP's float4)
 template <>
 struct Vec4<float> {
-  using type = float4;
+  using type = float __attribute__((ext_vector_type(4)));
 };
 template <>
 struct Vec4<int32_t> {
-  using type = int4;
+  using type = int32_t __attribute__((ext_vector_type(4)));
 };
 
 // Block of 256 threads computes a 128x128 tile of one batch entry; thread
 // (tx, ty) owns rows {tx*4..+3, 64+tx*4..+3} and the same pattern of columns.
 // VEC: A, B 16-byte aligned with lda, ldb, M, K multiples of 4.
 template <typename T, bool VEC>
-__device__ __forceinline__ void gemm_tile(const T* __restrict__ A, const T* __restrict__ B,
-                                          T* __restrict__ C, int M, int N, int K, int64_t lda,
+__device__ __forceinline__ void gemm_tile(const TBH_GLOBAL T* __restrict__ A,
+                                          const TBH_GLOBAL T* __restrict__ B,
+                                          TBH_GLOBAL T* __restrict__ C, int M, int N, int K, int64_t lda,
                                           int64_t ldb, int64_t ldc, int m0, int n0,
                                           T (*As)[BK][BM], T (*Bs)[BK][BN + BPAD]) {
   using V = typename Vec4<T>::type;
@@ -512,7 +607,7 @@ __device__ __forceinline__ void gemm_tile(const T* __restrict__ A, const T* __re
         const int m = m0 + (tid & 31) * 4, k = k0 + (tid >> 5) + 8 * e;
         V v;
         if (m < M && k < K)
-          v = *reinterpret_cast<const V*>(A + (int64_t)k * lda + m);
+          v = *reinterpret_cast<const TBH_GLOBAL V*>(A + (int64_t)k * lda + m);
         else
           v = V{pz, pz, pz, pz};
         ra[4 * e + 0] = v.x;
@@ -526,7 +621,7 @@ __device__ __forceinline__ void gemm_tile(const T* __restrict__ A, const T* __re
         const int k = k0 + (tid & 3) * 4, n = n0 + (tid >> 2) + 64 * e;
         V v;
         if (n < N && k < K)
-          v = *reinterpret_cast<const V*>(B + (int64_t)n * ldb + k);
+          v = *reinterpret_cast<const TBH_GLOBAL V*>(B + (int64_t)n * ldb + k);
         else
           v = V{pz, pz, pz, pz};
         rb[4 * e + 0] = v.x;
@@ -596,8 +691,10 @@ __device__ __forceinline__ void gemm_tile(const T* __restrict__ A, const T* __re
       for (int i = 0; i < 8; i++)
 #pragma unroll
         for (int j = 0; j < 8; j++)
-          acc[i][j] = Trop<T>::add(acc[i][j], Trop<T>::add(Trop<T>::mul(a0[i], b0[j]),
-                                                           Trop<T>::mul(a1[i], b1[j])));
+          acc[i][j] = max3(acc[i][j], Trop<T>::mul(a0[i], b0[j]), Trop<T>::mul(a1[i], b1[j]));
+      // one k pair at a time: keeps the LDS values of later pairs from being
+      // hoisted into registers (the 8x8 maxima already take 64)
+      __builtin_amdgcn_sched_barrier(0);
     }
     if (more) store(buf ^ 1);
     __syncthreads();
@@ -612,7 +709,7 @@ __device__ __forceinline__ void gemm_tile(const T* __restrict__ A, const T* __re
       for (int h = 0; h < 2; h++) {
         const int m = m0 + h * 64 + tx * 4;
         if (m < M)
-          *reinterpret_cast<V*>(C + (int64_t)n * ldc + m) =
+          *reinterpret_cast<TBH_GLOBAL V*>(C + (int64_t)n * ldc + m) =
               V{Trop<T>::canon(acc[4 * h][j]), Trop<T>::canon(acc[4 * h + 1][j]),
                 Trop<T>::canon(acc[4 * h + 2][j]), Trop<T>::canon(acc[4 * h + 3][j])};
       }
@@ -628,7 +725,7 @@ __device__ __forceinline__ void gemm_tile(const T* __restrict__ A, const T* __re
 
 // Grouped: prefix[i] is the first tile of GEMM step i; a block walks tiles.
 template <typename T, bool VEC>
-__global__ __launch_bounds__(256) void tbh_gemm_kernel(const GemmDesc* __restrict__ ops,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBH_GEMM_WAVES))) void tbh_gemm_kernel(const GemmDesc* __restrict__ ops,
                                                        const int64_t* __restrict__ prefix,
                                                        int nops, int64_t total) {
   __shared__ T As[2][BK][BM];
@@ -648,12 +745,48 @@ __global__ __launch_bounds__(256) void tbh_gemm_kernel(const GemmDesc* __restric
     const int64_t t = local / per;
     const int64_t rem = local - t * per;
     const int tm = (int)(rem % d->tiles_m), tn = (int)(rem / d->tiles_m);
-    const T* A = reinterpret_cast<const T*>(d->a) + t * d->sA;
-    const T* B = reinterpret_cast<const T*>(d->b) + t * d->sB;
-    T* C = reinterpret_cast<T*>(d->c) + t * d->sC;
+    const TBH_GLOBAL T* A = reinterpret_cast<const TBH_GLOBAL T*>(d->a) + t * d->sA;
+    const TBH_GLOBAL T* B = reinterpret_cast<const TBH_GLOBAL T*>(d->b) + t * d->sB;
+    TBH_GLOBAL T* C = reinterpret_cast<TBH_GLOBAL T*>(d->c) + t * d->sC;
     gemm_tile<T, VEC>(A, B, C, d->M, d->N, d->K, d->lda, d->ldb, d->ldc, tm * BM, tn * BN, As, Bs);
     __syncthreads();
   }
+}
+
+// ------------------------------------------------------ VALU probe -----
+// The GEMM inner loop's instruction mix with register operands only: per
+// iteration an 8 x 8 block of maxima takes two k values (128 adds, 64 max3
+// for 128 tropical MACs).  The empty asm marks a and b as changed so the
+// adds are not hoisted out of the loop; one store per thread keeps the
+// maxima live.
+template <typename T>
+__global__ __launch_bounds__(256) void tbh_valu_probe_kernel(T* __restrict__ out, int iters) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  T a0[8], a1[8], b0[8], b1[8], acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    a0[i] = (T)(tid & 7) + (T)i;
+    a1[i] = (T)((tid >> 3) & 7) + (T)i;
+    b0[i] = (T)(2 * i);
+    b1[i] = (T)(3 * i);
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[i][j] = Trop<T>::zero();
+  }
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        acc[i][j] = max3(acc[i][j], Trop<T>::mul(a0[i], b0[j]), Trop<T>::mul(a1[i], b1[j]));
+#pragma unroll
+    for (int i = 0; i < 8; i++) asm volatile("" : "+v"(a0[i]), "+v"(a1[i]), "+v"(b0[i]), "+v"(b1[i]));
+  }
+  T m = Trop<T>::zero();
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) m = Trop<T>::add(m, acc[i][j]);
+  out[tid] = m;
 }
 
 // ---------------------------------------------------------------- K3 -----
@@ -747,6 +880,14 @@ hipError_t launch_epilogue(int dtype, const EpiDesc* e, int64_t n, void* out, hi
     tbh_epi_kernel<float><<<grid, 256, 0, s>>>(e, n, reinterpret_cast<float*>(out));
   else
     tbh_epi_kernel<int32_t><<<grid, 256, 0, s>>>(e, n, reinterpret_cast<int32_t*>(out));
+  return hipGetLastError();
+}
+
+hipError_t launch_valu_probe(int dtype, void* out, int blocks, int iters, hipStream_t s) {
+  if (dtype == 0)
+    tbh_valu_probe_kernel<float><<<blocks, 256, 0, s>>>(static_cast<float*>(out), iters);
+  else
+    tbh_valu_probe_kernel<int32_t><<<blocks, 256, 0, s>>>(static_cast<int32_t*>(out), iters);
   return hipGetLastError();
 }
 

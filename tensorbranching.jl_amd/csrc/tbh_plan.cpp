@@ -156,47 +156,26 @@ struct Builder {
     op.nR = (int)R.size();
     auto inA = [&](int32_t l) { return pos_in(LA, l) >= 0; };
     auto inB = [&](int32_t l) { return pos_in(LB, l) >= 0; };
-    auto inR = [&](int32_t l) { return std::find(R.begin(), R.end(), l) != R.end(); };
-    int RA = 0, RB = 0;
-    for (int32_t l : R) RA += inA(l), RB += inB(l);
-    if (RA > kSubBits || RB > kSubBits) return fail(err, "EW step: reduced labels exceed a tile");
-    // tile labels: C's lowest positions, then enough of A's and B's lowest
-    // output labels that every operand is read contiguously, within the LDS
-    // budget; grown to >= 2^8 outputs where the output allows
+    // Ri: the reduced labels held in the sub-tensors (R's first |Ri|, i.e.
+    // the lowest A positions); the rest are outer reduced labels
+    std::vector<int32_t> Ri = R;
+    auto inRi = [&](int32_t l) { return std::find(Ri.begin(), Ri.end(), l) != Ri.end(); };
     std::vector<int32_t> T;
-    int nTA = 0, nTB = 0;
-    auto try_add = [&](int32_t l) {
-      if (std::find(T.begin(), T.end(), l) != T.end()) return true;
-      if ((int)T.size() >= kTileBits) return false;
-      const int ia = inA(l), ib = inB(l);
-      if (nTA + ia + RA > kSubBits || nTB + ib + RB > kSubBits) return false;
-      T.push_back(l);
-      nTA += ia;
-      nTB += ib;
-      return true;
-    };
-    auto cover = [&](const std::vector<int32_t>& L, int want) {
-      int got = 0;
-      for (int32_t l : L) {
-        if (got >= want) break;
-        if (inR(l) || try_add(l))
-          got++;
-        else
-          break;
-      }
-    };
     // direct step: the tile is C's lowest min(nP, 10) positions and the
-    // operands are read straight from memory; taken when each operand is
-    // small (cache resident) or aligned: its lowest positions are lane labels
-    // (C positions 0..5) or reduced labels, so the loads are coalesced
+    // operands are read straight from memory; taken when every reduced label
+    // fits the sub-tensor budget and each operand is small (cache resident)
+    // or aligned: its lowest positions are lane labels (C positions 0..5) or
+    // reduced labels, so the loads are coalesced
     {
+      int RA = 0, RB = 0;
+      for (int32_t l : R) RA += inA(l), RB += inB(l);
       const int nTd = std::min(op.nP, 10);
       int na = 0, nb = 0;
       for (int p = 0; p < nTd; p++) na += inA(LC[p]), nb += inB(LC[p]);
       auto aligned = [&](const std::vector<int32_t>& L) {
         for (int p = 0; p < std::min<int>(5, (int)L.size()); p++) {
           const int pc = pos_in(LC, L[p]);
-          if (!inR(L[p]) && !(pc >= 0 && pc < std::min(6, nTd))) return false;
+          if (!inRi(L[p]) && !(pc >= 0 && pc < std::min(6, nTd))) return false;
         }
         return true;
       };
@@ -208,13 +187,68 @@ struct Builder {
       }
     }
     if (!op.direct) {
-      for (int p = 0; p < std::min(5, op.nP); p++) try_add(LC[p]);
-      cover(LA, 5);
-      cover(LB, 5);
-      // grow the tile while the budget allows: a tile's loads are its only
-      // latency hiding, so it should carry up to 2^12 outputs / sub-tensor
-      // entries
-      for (int p = 5; p < op.nP; p++) try_add(LC[p]);
+      // tiled: for r_in inner reduced labels, the tile takes C's lowest
+      // positions, then enough of A's and B's lowest output labels that
+      // every operand is read contiguously, then grows toward 2^12 outputs
+      // within the LDS budget (a tile's loads are its only latency hiding)
+      auto select = [&](int r_in, std::vector<int32_t>& Tt) {
+        Ri.assign(R.begin(), R.begin() + r_in);
+        int RA = 0, RB = 0;
+        for (int32_t l : Ri) RA += inA(l), RB += inB(l);
+        Tt.clear();
+        if (RA > kSubBits || RB > kSubBits) return -1;
+        int nTA = 0, nTB = 0;
+        auto try_add = [&](int32_t l) {
+          if (std::find(Tt.begin(), Tt.end(), l) != Tt.end()) return true;
+          if ((int)Tt.size() >= kTileBits) return false;
+          const int ia = inA(l), ib = inB(l);
+          if (nTA + ia + RA > kSubBits || nTB + ib + RB > kSubBits) return false;
+          Tt.push_back(l);
+          nTA += ia;
+          nTB += ib;
+          return true;
+        };
+        auto cover = [&](const std::vector<int32_t>& L, int want) {
+          int got = 0;
+          for (int32_t l : L) {
+            if (got >= want) break;
+            if (inRi(l) || (pos_in(LC, l) >= 0 && try_add(l)))
+              got++;
+            else
+              break;
+          }
+        };
+        for (int p = 0; p < std::min(5, op.nP); p++) try_add(LC[p]);
+        cover(LA, 5);
+        cover(LB, 5);
+        for (int p = 5; p < op.nP; p++) try_add(LC[p]);
+        return RA + nTA + (RB + nTB) * 16;  // sub-tensor bits of A, B
+      };
+      // modelled cycles per MAC of one workgroup: per chunk of the outer
+      // reduced loop a fixed cost (two barriers, load latency) plus staging
+      // (4 cycles per staged element per wave), plus 16 cycles per MAC
+      // column of a wave; a tile below 256 outputs leaves threads idle
+      auto cost = [&](int nt, int r_in, int sa, int sb) {
+        const double cnt = std::ldexp(1.0, std::max(0, nt - 8));
+        const double chunk = 400.0 + 16.0 * (std::ldexp(1.0, sa) + std::ldexp(1.0, sb)) / 256.0 +
+                             16.0 * cnt * std::ldexp(1.0, r_in);
+        return chunk / std::ldexp(1.0, nt + r_in);
+      };
+      int best_r = -1;
+      double best_c = 0;
+      for (int r_in = (int)R.size(); r_in >= 0; r_in--) {
+        std::vector<int32_t> Tt;
+        const int bits = select(r_in, Tt);
+        if (bits < 0) continue;
+        const double c = cost((int)Tt.size(), r_in, bits % 16, b >= 0 ? bits / 16 : -100);
+        if (best_r < 0 || c < best_c * 0.999) {
+          best_r = r_in;
+          best_c = c;
+        }
+      }
+      if (best_r < 0 || (int)R.size() - best_r > kMaxReduce)
+        return fail(err, "EW step: reduced labels exceed a tile");
+      select(best_r, T);
     }
     std::sort(T.begin(), T.end(),
               [&](int32_t x, int32_t y) { return pos_in(LC, x) < pos_in(LC, y); });
@@ -223,26 +257,32 @@ struct Builder {
       if (std::find(T.begin(), T.end(), l) == T.end()) O.push_back(l);
     std::vector<int32_t> SA, SB;  // sub-tensor labels in operand order
     for (int32_t l : LA)
-      if (inR(l) || std::find(T.begin(), T.end(), l) != T.end()) SA.push_back(l);
+      if (inRi(l) || std::find(T.begin(), T.end(), l) != T.end()) SA.push_back(l);
     for (int32_t l : LB)
-      if (inR(l) || std::find(T.begin(), T.end(), l) != T.end()) SB.push_back(l);
+      if (inRi(l) || std::find(T.begin(), T.end(), l) != T.end()) SB.push_back(l);
+    std::vector<int32_t> Ro(R.begin() + Ri.size(), R.end());  // outer reduced
     EwDesc& d = op.ew;
     std::memset(&d, -1, sizeof d);
     d.a = d.b = d.c = 0;
     d.nT = (int)T.size();
     d.nO = (int)O.size();
-    d.nR = op.nR;
+    d.nR = (int)Ri.size();
+    d.nRo = (int)Ro.size();
     d.nSA = (int)SA.size();
     d.nSB = (int)SB.size();
     d.tiles = 1 << d.nO;
+    for (int i = 0; i < d.nRo; i++) {
+      d.qA[i] = (int8_t)pos_in(LA, Ro[i]);
+      d.qB[i] = (int8_t)pos_in(LB, Ro[i]);
+    }
     for (int k = 0; k < d.nT; k++) {
       d.tC[k] = (int8_t)pos_in(LC, T[k]);
       d.tA[k] = (int8_t)pos_in(SA, T[k]);
       d.tB[k] = (int8_t)pos_in(SB, T[k]);
     }
-    for (int i = 0; i < op.nR; i++) {
-      d.rA[i] = (int8_t)pos_in(SA, R[i]);
-      d.rB[i] = (int8_t)pos_in(SB, R[i]);
+    for (int i = 0; i < d.nR; i++) {
+      d.rA[i] = (int8_t)pos_in(SA, Ri[i]);
+      d.rB[i] = (int8_t)pos_in(SB, Ri[i]);
     }
     for (int k = 0; k < d.nSA; k++) d.sA[k] = (int8_t)pos_in(LA, SA[k]);
     for (int k = 0; k < d.nSB; k++) d.sB[k] = (int8_t)pos_in(LB, SB[k]);
@@ -359,9 +399,10 @@ int verify_plan(const SlicePlan& P, std::string* err) {
     if (op.kind == kEw) {
       const EwDesc& d = op.ew;
       if (op.nP != rc || d.nT + d.nO != rc || d.nT < 0 || d.nT > kTileBits || d.nO < 0 ||
-          d.nR != op.nR || d.nR < 0 || d.nR > kMaxReduce || d.nSA < 0 || d.nSA > kSubBits ||
-          d.nSB < 0 || d.nSB > kSubBits || (op.b < 0 && d.nSB != 0) ||
-          d.tiles != (1 << d.nO) || op.tiles != d.tiles || (op.direct && d.nT > 10))
+          d.nR + d.nRo != op.nR || d.nR < 0 || d.nR > kMaxReduce || d.nRo < 0 ||
+          d.nRo > kMaxReduce || d.nSA < 0 || d.nSA > kSubBits || d.nSB < 0 ||
+          d.nSB > kSubBits || (op.b < 0 && d.nSB != 0) || d.tiles != (1 << d.nO) ||
+          op.tiles != d.tiles || (op.direct && (d.nT > 10 || d.nRo != 0)))
         return bad("EW shape");
       // every position of A, B, C is covered exactly once, every sub-tensor
       // bit by exactly one tile or reduced bit, and all indices in range
@@ -384,6 +425,14 @@ int verify_plan(const SlicePlan& P, std::string* err) {
           continue;
         }
         if (!take(d.rA[i], d.nSA, usa) || !take(d.rB[i], d.nSB, usb)) return bad("EW reduced bits");
+      }
+      for (int i = 0; i < kMaxReduce; i++) {
+        if (i >= d.nRo) {
+          if (d.qA[i] != -1 || d.qB[i] != -1) return bad("EW outer reduced padding");
+          continue;
+        }
+        if ((d.qA[i] < 0 && d.qB[i] < 0) || !take(d.qA[i], ra, ua) || !take(d.qB[i], rb, ub))
+          return bad("EW outer reduced bits");
       }
       for (int k = 0; k < kSubBits; k++) {
         if (k >= d.nSA && d.sA[k] != -1) return bad("EW sub-tensor padding");

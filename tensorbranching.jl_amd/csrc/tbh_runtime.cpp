@@ -322,7 +322,13 @@ int direct_grid(tbh_ctx* ctx, int64_t total, int chunk) {
 // Waves (direct) and workgroups (tiled) take small chunks of the tile space
 // round-robin, which spreads expensive steps evenly; long contiguous ranges
 // measured slower (costly steps cluster in a few ranges).
-int direct_chunk(tbh_ctx*, int64_t) { return 4; }
+int direct_chunk(tbh_ctx*, int64_t) {
+  static const int c = [] {
+    const char* e = std::getenv("TBH_DIRECT_CHUNK");  // tuning sweeps
+    return e ? std::max(1, std::atoi(e)) : 4;
+  }();
+  return c;
+}
 int ew_chunk(tbh_ctx*, int64_t) { return 2; }
 
 }  // namespace
@@ -1078,11 +1084,55 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode, tbh_s
     if (op.kind == kEw) {
       for (int k = 0; k < op.ew.nSA && op.ew.sA[k] == k; k++) s.lanes_a++;
       for (int k = 0; k < op.ew.nSB && op.ew.sB[k] == k; k++) s.lanes_b++;
+      s.tile_bits = op.ew.nT;
+      s.outer_reduced = op.ew.nRo;
+      s.direct = op.direct ? 1 : 0;
     }
     s.macs = op.macs;
     s.bytes = op.bytes;
   }
   return n;
+}
+
+int tbh_valu_peak(tbh_ctx* ctx, tbh_dtype dtype, double* tropical_ops_per_s) {
+  if (!ctx || !tropical_ops_per_s) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  if (dtype != TBH_F32 && dtype != TBH_I32) {
+    set_error("unknown dtype");
+    return TBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int blocks = ctx->num_cus * 8, iters = 4096;
+  const size_t bytes = (size_t)blocks * 256 * 4;
+  const int64_t off = ctx->pool.alloc(bytes);
+  if (off < 0) {
+    set_error("pool exhausted");
+    return TBH_ENOMEM;
+  }
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  if (e == hipSuccess) e = launch_valu_probe(dtype, ctx->base + off, blocks, 64, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
+  if (e == hipSuccess) e = launch_valu_probe(dtype, ctx->base + off, blocks, iters, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  ctx->pool.free(off);
+  if (e != hipSuccess) {
+    set_error(std::string("valu probe: ") + hipGetErrorString(e));
+    return TBH_EDEVICE;
+  }
+  // 64 maxima x 2 k values per iteration per thread, 2 ops per MAC
+  const double ops = 2.0 * 128.0 * iters * (double)blocks * 256.0;
+  *tropical_ops_per_s = ops / (ms * 1e-3);
+  return 0;
 }
 
 int tbh_get_stats(tbh_ctx* ctx, tbh_stats* out) {

@@ -30,6 +30,7 @@ hipError_t launch_gemm(int dtype, bool vec, const GemmDesc* ops, const int64_t* 
                        int64_t total, int grid, hipStream_t s);
 hipError_t launch_leaves(int dtype, const LeafDesc* leaves, int64_t n, hipStream_t s);
 hipError_t launch_epilogue(int dtype, const EpiDesc* e, int64_t n, void* out, hipStream_t s);
+hipError_t launch_valu_probe(int dtype, void* out, int blocks, int iters, hipStream_t s);
 
 // First-fit allocator over one device allocation (offsets, 256-byte units).
 class Pool {

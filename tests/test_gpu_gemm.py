@@ -56,3 +56,24 @@ def test_gemm_large_rank_one(backend):
     C = backend.maxplus_gemm(A, B)
     expect = u[:, None] + (v[:, None] + B).max(axis=0)[None, :]
     assert np.array_equal(C, expect)
+
+
+def test_valu_peak_bounds_the_gemm(backend):
+    """The measured VALU ceiling of the max-plus mix is positive and the
+    register-blocked GEMM stays below it (it issues the same mix plus
+    loads)."""
+    import time
+    for dt in (np.float32, np.int32):
+        peak = backend.valu_peak(dt)
+        assert 5e12 < peak < 4e14, peak
+    n = 2048
+    rng = np.random.default_rng(1)
+    A = np.asfortranarray(rng.integers(0, 1024, (n, n)).astype(np.float32))
+    B = np.asfortranarray(rng.integers(0, 1024, (n, n)).astype(np.float32))
+    backend.maxplus_gemm(A, B)
+    backend.set_profiling(True)
+    backend.reset_stats()
+    backend.maxplus_gemm(A, B)
+    ms = backend.stats()["kind_ms"][2]
+    backend.set_profiling(False)
+    assert 2 * n ** 3 / (ms * 1e-3) < 1.1 * backend.valu_peak(np.float32)

@@ -193,6 +193,10 @@ typedef struct {
   double kind_bytes[3];     /* algorithmic bytes of the steps timed         */
   double kind_macs[3];      /* executed (output x reduced) configurations   */
   uint64_t kind_launches[3];
+  /* host wall time of contract calls (ms): [0] converting and planning the
+   * slices, [1] building and packing step descriptors, [2] issuing the
+   * uploads and launches, [3] waiting for the stream */
+  double host_ms[4];
 } tbh_stats;
 /* Time every launch with HIP events (adds a little host overhead). */
 int tbh_set_profiling(tbh_ctx* ctx, int on);

@@ -36,7 +36,7 @@ class TbhStats(C.Structure):
                 ("total_algo_ops", C.c_double), ("pool_bytes", C.c_size_t),
                 ("pool_peak_bytes", C.c_size_t), ("kind_ms", C.c_double * 3),
                 ("kind_bytes", C.c_double * 3), ("kind_macs", C.c_double * 3),
-                ("kind_launches", C.c_uint64 * 3)]
+                ("kind_launches", C.c_uint64 * 3), ("host_ms", C.c_double * 4)]
 
 
 class TbhStepInfo(C.Structure):

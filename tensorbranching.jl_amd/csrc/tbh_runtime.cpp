@@ -17,6 +17,10 @@
 
 namespace tbh {
 
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 static thread_local std::string g_err;
 void set_error(const std::string& m) { g_err = m; }
 
@@ -334,14 +338,50 @@ int ew_chunk(tbh_ctx*, int64_t) { return 2; }
 }  // namespace
 
 // ------------------------------------------------------ BATCHED ----------
+namespace {
+
+int op_kind(const POp& op) { return op.kind == kGemm ? 2 : (op.direct ? 0 : 1); }
+int64_t op_tiles(const POp& op) {
+  return op.kind == kGemm ? (int64_t)((op.M + 127) / 128) * ((op.N + 127) / 128) * op.batch
+                          : op.tiles;
+}
+size_t desc_size(int kind) { return kind == 2 ? sizeof(GemmDesc) : sizeof(EwDesc); }
+
+// Runs f(i) for i in [0, n) on up to `threads` host threads.
+template <typename F>
+void parallel_for(size_t n, int threads, F f) {
+  const int nt = (int)std::min<size_t>(std::max(1, threads), std::max<size_t>(1, n / 16));
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; i++) f(i, 0);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; t++)
+    pool.emplace_back([&, t]() {
+      for (size_t i; (i = next.fetch_add(1)) < n;) f(i, t);
+    });
+  for (auto& th : pool) th.join();
+}
+
+}  // namespace
+
+// A batch is packed in two passes.  Pass 1 (serial, integer work only)
+// counts the steps of every (kernel kind, level), gives each slice its first
+// index in each, and builds the tile prefixes; pass 2 (host threads, one
+// slice at a time) writes every descriptor straight into its final place in
+// the pinned staging buffer, which goes to the device in one copy.
 int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*>& slices,
                     const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* results) {
   const size_t n = slices.size();
+  const uint32_t zero_bits = dtype == TBH_F32 ? f2b(0.0f) : 0u;
+  const bool per_step = ctx->profiling && std::getenv("TBH_PROFILE_STEPS");
   size_t i0 = 0;
   while (i0 < n) {
-    // admit slices while their arenas fit (leave room for descriptors)
-    // (descriptors are counted per slice below; the reserve covers the
-    // pool's first-fit fragmentation: 1/16 of the free block, <= 64 MiB)
+    auto t_pack = std::chrono::steady_clock::now();
+    // admit slices while their arenas fit (descriptors are counted per
+    // slice; the reserve covers the pool's first-fit fragmentation: 1/16 of
+    // the free block, <= 64 MiB)
     size_t budget = ctx->pool.largest_free();
     budget -= std::min<size_t>((size_t)64 << 20, budget / 16);
     size_t i1 = i0, arena = 0, ndesc_bytes = 0;
@@ -358,129 +398,73 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
       set_error("slice arena (" + std::to_string(arena) + " bytes) does not fit the pool");
       return TBH_ENOMEM;
     }
+    const size_t nb = i1 - i0;
     int64_t arena_off = ctx->pool.alloc(std::max<size_t>(arena, 256));
     if (arena_off < 0) {
       set_error("pool exhausted");
       return TBH_ENOMEM;
     }
     const uint64_t arena_base = (uint64_t)(ctx->base + arena_off);
-    // per-slice bases
-    std::vector<uint64_t> sbase(i1 - i0);
+    std::vector<uint64_t> sbase(nb);
+    int H = 0;
     {
       uint64_t acc = 0;
-      for (size_t i = i0; i < i1; i++) {
-        sbase[i - i0] = arena_base + acc;
-        acc += plans[i]->empty ? 0 : (uint64_t)plans[i]->arena_bytes;
+      for (size_t i = 0; i < nb; i++) {
+        sbase[i] = arena_base + acc;
+        acc += plans[i0 + i]->empty ? 0 : (uint64_t)plans[i0 + i]->arena_bytes;
+        H = std::max(H, plans[i0 + i]->nlevels);
       }
     }
-    int H = 0;
-    for (size_t i = i0; i < i1; i++) H = std::max(H, plans[i]->nlevels);
-    // gather steps by level
-    std::vector<std::vector<EwDesc>> ew(H + 1), ews(H + 1);
-    std::vector<std::vector<GemmDesc>> gm(H + 1);
-    std::vector<double> lvl_bytes[3], lvl_macs[3];  // per kernel kind and level
-    std::vector<std::vector<double>> ew_bytes(H + 1), ew_macs(H + 1);  // tiled steps
-    std::vector<std::vector<std::pair<int, int>>> ew_ranks(H + 1);
-    std::vector<std::vector<double>> ews_bytes(H + 1), ews_macs(H + 1);  // direct steps
-    std::vector<std::vector<std::pair<int, int>>> ews_ranks(H + 1);
-    std::vector<std::vector<double>> gm_bytes(H + 1), gm_macs(H + 1);  // GEMM steps
-    for (int k = 0; k < 3; k++) {
-      lvl_bytes[k].assign(H + 1, 0.0);
-      lvl_macs[k].assign(H + 1, 0.0);
-    }
-    std::vector<LeafDesc> leaves;
-    std::vector<EpiDesc> epi(i1 - i0);
-    for (size_t i = i0; i < i1; i++) {
-      const SlicePlan& P = *plans[i];
-      const uint64_t b = sbase[i - i0];
-      EpiDesc& e = epi[i - i0];
-      std::memset(&e, 0, sizeof e);
-      e.rbits = add_r ? slices[i]->rbits : (dtype == TBH_F32 ? f2b(0.0f) : 0u);
-      if (P.empty) {
-        e.root = 0;
-        if (!add_r) e.rbits = dtype == TBH_F32 ? f2b(0.0f) : 0u;
-        continue;
-      }
-      e.root = b + (uint64_t)P.tensors[P.root].off;
-      for (const PLeaf& L : P.leaves) {
-        LeafDesc d;
-        d.p = b + (uint64_t)P.tensors[L.tensor].off;
-        d.kind = L.kind;
-        d.wbits = (int32_t)L.wbits;
-        leaves.push_back(d);
-      }
+    const int W = 3 * (H + 1);  // (kind, level) slots
+    auto slot = [H](int k, int L) { return k * (H + 1) + L; };
+    // pass 1
+    std::vector<int32_t> cnt(W, 0), start(nb * W);
+    std::vector<std::vector<int64_t>> pfx(W, std::vector<int64_t>(1, 0));
+    std::vector<double> lvl_bytes(W, 0.0), lvl_macs(W, 0.0);
+    std::vector<std::vector<std::pair<double, double>>> step_work(per_step ? W : 0);
+    std::vector<int64_t> leaf_start(nb + 1, 0);
+    for (size_t i = 0; i < nb; i++) {
+      const SlicePlan& P = *plans[i0 + i];
+      std::memcpy(&start[i * W], cnt.data(), W * sizeof(int32_t));
+      leaf_start[i + 1] = leaf_start[i] + (P.empty ? 0 : (int64_t)P.leaves.size());
+      if (P.empty) continue;
       for (const POp& op : P.ops) {
-        const int kind = op.kind == kGemm ? 2 : (op.direct ? 0 : 1);
+        const int w = slot(op_kind(op), op.level);
+        cnt[w]++;
+        pfx[w].push_back(pfx[w].back() + op_tiles(op));
+        lvl_bytes[w] += op.bytes;
+        lvl_macs[w] += op.macs;
+        if (per_step) step_work[w].push_back({op.bytes, op.macs});
         if (op.kind == kEw) {
-          (op.direct ? ews : ew)[op.level].push_back(make_ew(op, P.tensors, b));
-          auto& xb = op.direct ? ews_bytes : ew_bytes;
-          auto& xm = op.direct ? ews_macs : ew_macs;
-          auto& xr = op.direct ? ews_ranks : ew_ranks;
-          xb[op.level].push_back(op.bytes);
-          xm[op.level].push_back(op.macs);
-          xr[op.level].push_back({P.tensors[op.a].rank, op.b >= 0 ? P.tensors[op.b].rank : -1});
+          ctx->stats.ew_ops++;
+          ctx->stats.ew_ops_algo_bytes += op.bytes;
         } else {
-          gm[op.level].push_back(make_gemm(op, P.tensors, b));
-          gm_bytes[op.level].push_back(op.bytes);
-          gm_macs[op.level].push_back(op.macs);
+          ctx->stats.gemm_ops++;
+          ctx->stats.gemm_ops_algo_ops += 2 * op.macs;
         }
-        lvl_bytes[kind][op.level] += op.bytes;
-        lvl_macs[kind][op.level] += op.macs;
-        ctx->stats.ew_ops += op.kind == kEw;
-        ctx->stats.gemm_ops += op.kind == kGemm;
-        if (op.kind == kEw) ctx->stats.ew_ops_algo_bytes += op.bytes;
-        if (op.kind == kGemm) ctx->stats.gemm_ops_algo_ops += 2 * op.macs;
       }
       ctx->stats.total_algo_ops += 2 * P.node_macs;
       ctx->stats.leaf_tensors += P.leaves.size();
     }
-    // pack
-    Pack pk;
-    const size_t o_leaves = pk.put(leaves.data(), leaves.size() * sizeof(LeafDesc));
-    const size_t o_epi = pk.put(epi.data(), epi.size() * sizeof(EpiDesc));
-    const size_t o_res = pk.reserve(4 * (i1 - i0));
-    struct LvlOff {
-      size_t ew, ewp, ews, ewsp, gm, gmp, one, onew, onegm;
-      int64_t ew_total, ews_total, gm_total;
+    const int64_t nleaves = leaf_start[nb];
+    // layout of the upload (256-byte aligned arrays)
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      const size_t o = (off + 255) / 256 * 256;
+      off = o + bytes;
+      return o;
     };
-    const bool per_step = ctx->profiling && std::getenv("TBH_PROFILE_STEPS");
-    std::vector<LvlOff> lo(H + 1);
-    for (int L = 1; L <= H; L++) {
-      if (per_step) {
-        std::vector<int64_t> one, onew;
-        for (const EwDesc& d : ew[L]) {
-          one.push_back(0);
-          one.push_back(d.tiles);
-        }
-        for (const EwDesc& d : ews[L]) {
-          onew.push_back(0);
-          onew.push_back(d.tiles);
-        }
-        lo[L].one = pk.put(one.data(), one.size() * sizeof(int64_t));
-        lo[L].onew = pk.put(onew.data(), onew.size() * sizeof(int64_t));
-        std::vector<int64_t> onegm;
-        for (const GemmDesc& d : gm[L]) {
-          onegm.push_back(0);
-          onegm.push_back(gemm_tiles(d));
-        }
-        lo[L].onegm = pk.put(onegm.data(), onegm.size() * sizeof(int64_t));
-      }
-      std::vector<int64_t> pe(ew[L].size() + 1, 0), pg(gm[L].size() + 1, 0);
-      for (size_t k = 0; k < ew[L].size(); k++) pe[k + 1] = pe[k] + ew[L][k].tiles;
-      for (size_t k = 0; k < gm[L].size(); k++) pg[k + 1] = pg[k] + gemm_tiles(gm[L][k]);
-      lo[L].ew = pk.put(ew[L].data(), ew[L].size() * sizeof(EwDesc));
-      lo[L].ewp = pk.put(pe.data(), pe.size() * sizeof(int64_t));
-      lo[L].ews = pk.put(ews[L].data(), ews[L].size() * sizeof(EwDesc));
-      std::vector<int64_t> pd(ews[L].size() + 1, 0);
-      for (size_t k = 0; k < ews[L].size(); k++) pd[k + 1] = pd[k] + ews[L][k].tiles;
-      lo[L].ewsp = pk.put(pd.data(), pd.size() * sizeof(int64_t));
-      lo[L].ews_total = pd.back();
-      lo[L].gm = pk.put(gm[L].data(), gm[L].size() * sizeof(GemmDesc));
-      lo[L].gmp = pk.put(pg.data(), pg.size() * sizeof(int64_t));
-      lo[L].ew_total = pe.back();
-      lo[L].gm_total = pg.back();
+    const size_t o_leaves = take((size_t)nleaves * sizeof(LeafDesc));
+    const size_t o_epi = take(nb * sizeof(EpiDesc));
+    const size_t o_res = take(4 * nb);
+    std::vector<size_t> o_desc(W), o_pfx(W), o_one(W);
+    for (int w = 0; w < W; w++) {
+      if (!cnt[w]) continue;
+      o_desc[w] = take((size_t)cnt[w] * desc_size(w / (H + 1)));
+      o_pfx[w] = take((size_t)(cnt[w] + 1) * sizeof(int64_t));
+      if (per_step) o_one[w] = take((size_t)cnt[w] * 2 * sizeof(int64_t));
     }
-    const size_t bytes = pk.buf.size();
+    const size_t bytes = off;
     int64_t doff = ctx->pool.alloc(bytes);
     if (doff < 0) {
       ctx->pool.free(arena_off);
@@ -494,7 +478,51 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
       ctx->pool.free(arena_off);
       return rc;
     }
-    std::memcpy(ctx->staging.host, pk.buf.data(), bytes);
+    char* hb = static_cast<char*>(ctx->staging.host);
+    for (int w = 0; w < W; w++) {
+      if (!cnt[w]) continue;
+      std::memcpy(hb + o_pfx[w], pfx[w].data(), pfx[w].size() * sizeof(int64_t));
+      if (per_step) {
+        int64_t* one = reinterpret_cast<int64_t*>(hb + o_one[w]);
+        for (int32_t k = 0; k < cnt[w]; k++) {
+          one[2 * k] = 0;
+          one[2 * k + 1] = pfx[w][k + 1] - pfx[w][k];
+        }
+      }
+    }
+    // pass 2
+    std::vector<std::vector<int32_t>> local(std::max(1, ctx->plan_threads),
+                                            std::vector<int32_t>(W));
+    parallel_for(nb, ctx->plan_threads, [&](size_t i, int t) {
+      const SlicePlan& P = *plans[i0 + i];
+      const uint64_t b = sbase[i];
+      EpiDesc e;
+      std::memset(&e, 0, sizeof e);
+      e.rbits = add_r ? slices[i0 + i]->rbits : zero_bits;
+      e.root = P.empty ? 0 : b + (uint64_t)P.tensors[P.root].off;
+      std::memcpy(hb + o_epi + i * sizeof(EpiDesc), &e, sizeof e);
+      if (P.empty) return;
+      LeafDesc* ld = reinterpret_cast<LeafDesc*>(hb + o_leaves) + leaf_start[i];
+      for (const PLeaf& L : P.leaves) {
+        ld->p = b + (uint64_t)P.tensors[L.tensor].off;
+        ld->kind = L.kind;
+        ld->wbits = (int32_t)L.wbits;
+        ld++;
+      }
+      std::vector<int32_t>& loc = local[t];
+      std::fill(loc.begin(), loc.end(), 0);
+      const int32_t* st = &start[i * W];
+      for (const POp& op : P.ops) {
+        const int k = op_kind(op), w = slot(k, op.level);
+        const int32_t idx = st[w] + loc[w]++;
+        if (k == 2)
+          reinterpret_cast<GemmDesc*>(hb + o_desc[w])[idx] = make_gemm(op, P.tensors, b);
+        else
+          reinterpret_cast<EwDesc*>(hb + o_desc[w])[idx] = make_ew(op, P.tensors, b);
+      }
+    });
+    ctx->stats.host_ms[1] += ms_since(t_pack);
+    auto t_issue = std::chrono::steady_clock::now();
     auto cleanup = [&]() {
       ctx->pool.free(doff);
       ctx->pool.free(arena_off);
@@ -506,106 +534,81 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
       }
       return false;
     };
-    if (check(hipMemcpyAsync(dbase, ctx->staging.host, bytes, hipMemcpyHostToDevice, ctx->stream),
-              "upload")) {
+    if (check(hipMemcpyAsync(dbase, hb, bytes, hipMemcpyHostToDevice, ctx->stream), "upload")) {
       (void)hipStreamSynchronize(ctx->stream);
       cleanup();
       return TBH_EDEVICE;
     }
-    hipError_t e = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), (int64_t)leaves.size(),
-                                 ctx->stream);
-    ctx->stats.launches += leaves.empty() ? 0 : 1;
+    hipError_t e = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), nleaves, ctx->stream);
+    ctx->stats.launches += nleaves ? 1 : 0;
     const bool prof = ctx->profiling;
     hipEvent_t ev = nullptr;
+    // one launch per kernel kind and level (direct EW, tiled EW, GEMM); in
+    // the per-step profiling mode every step alone, tagged with its shape
+    auto launch = [&](int k, const char* descs, const int64_t* prefix, int nops, int64_t total) {
+      if (k == 0)
+        return launch_ew_direct(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
+                                direct_grid(ctx, total, direct_chunk(ctx, total)),
+                                direct_chunk(ctx, total), ctx->stream);
+      if (k == 1)
+        return launch_ew(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
+                         ew_grid(ctx, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
+                         ctx->stream);
+      return launch_gemm(dtype, true, reinterpret_cast<const GemmDesc*>(descs), prefix, nops, total,
+                         gemm_grid(ctx, total), ctx->stream);
+    };
     for (int L = 1; L <= H && e == hipSuccess; L++) {
-      if (!ews[L].empty() && per_step) {
-        for (size_t k = 0; k < ews[L].size() && e == hipSuccess; k++) {
-          const EwDesc& d = ews[L][k];
-          if (d.nO < 8) continue;  // only the big ones
-          ev = ctx->prof.begin(ctx->stream);
-          e = launch_ew_direct(dtype, (const EwDesc*)(dbase + lo[L].ews) + k,
-                               (const int64_t*)(dbase + lo[L].onew) + 2 * k, 1, d.tiles,
-                               direct_grid(ctx, d.tiles, direct_chunk(ctx, d.tiles)),
-                               direct_chunk(ctx, d.tiles), ctx->stream);
-          char tag[128];
-          std::snprintf(tag, sizeof tag, "nT=%d nO=%d nR=%d nSA=%d nSB=%d ra=%d rb=%d", d.nT,
-                        d.nO, d.nR, d.nSA, d.nSB, ews_ranks[L][k].first, ews_ranks[L][k].second);
-          ctx->prof.end(ctx->stream, ev, 0, L, 1, ews_bytes[L][k], ews_macs[L][k], tag);
+      for (int k = 0; k < 3 && e == hipSuccess; k++) {
+        const int w = slot(k, L);
+        if (!cnt[w]) continue;
+        const int64_t total = pfx[w].back();
+        if (per_step) {
+          for (int32_t q = 0; q < cnt[w] && e == hipSuccess; q++) {
+            char tag[160];
+            if (k == 2) {
+              const GemmDesc& d = reinterpret_cast<const GemmDesc*>(hb + o_desc[w])[q];
+              std::snprintf(tag, sizeof tag, "M=%d N=%d K=%d batch=%d", d.M, d.N, d.K, d.batch);
+            } else {
+              const EwDesc& d = reinterpret_cast<const EwDesc*>(hb + o_desc[w])[q];
+              if (k == 0 && d.nO < 8) continue;  // only the big direct steps
+              std::snprintf(tag, sizeof tag, "nT=%d nO=%d nR=%d nRo=%d nSA=%d nSB=%d", d.nT, d.nO,
+                            d.nR, d.nRo, d.nSA, d.nSB);
+            }
+            const int64_t tiles = pfx[w][q + 1] - pfx[w][q];
+            ev = ctx->prof.begin(ctx->stream);
+            e = launch(k, dbase + o_desc[w] + q * desc_size(k),
+                       reinterpret_cast<const int64_t*>(dbase + o_one[w]) + 2 * q, 1, tiles);
+            ctx->prof.end(ctx->stream, ev, k, L, 1, step_work[w][q].first, step_work[w][q].second,
+                          tag);
+            ctx->stats.launches++;
+          }
+          continue;
         }
-      } else if (!ews[L].empty()) {
         if (prof) ev = ctx->prof.begin(ctx->stream);
-        e = launch_ew_direct(dtype, (const EwDesc*)(dbase + lo[L].ews),
-                             (const int64_t*)(dbase + lo[L].ewsp), (int)ews[L].size(),
-                             lo[L].ews_total,
-                             direct_grid(ctx, lo[L].ews_total, direct_chunk(ctx, lo[L].ews_total)),
-                             direct_chunk(ctx, lo[L].ews_total), ctx->stream);
-        if (prof)
-          ctx->prof.end(ctx->stream, ev, 0, L, (int)ews[L].size(), lvl_bytes[0][L], lvl_macs[0][L]);
-        ctx->stats.launches++;
-      }
-      if (e == hipSuccess && lo[L].ew_total > 0 && per_step) {
-        // profiling of single steps: each tiled step alone, logged with its shape
-        for (size_t k = 0; k < ew[L].size() && e == hipSuccess; k++) {
-          const EwDesc& d = ew[L][k];
-          ev = ctx->prof.begin(ctx->stream);
-          e = launch_ew(dtype, (const EwDesc*)(dbase + lo[L].ew) + k,
-                        (const int64_t*)(dbase + lo[L].one) + 2 * k, 1, d.tiles,
-                        ew_grid(ctx, d.tiles, ew_chunk(ctx, d.tiles)), ew_chunk(ctx, d.tiles),
-                        ctx->stream);
-          char tag[128];
-          std::snprintf(tag, sizeof tag, "nT=%d nO=%d nR=%d nSA=%d nSB=%d ra=%d rb=%d", d.nT,
-                        d.nO, d.nR, d.nSA, d.nSB, ew_ranks[L][k].first, ew_ranks[L][k].second);
-          ctx->prof.end(ctx->stream, ev, 1, L, 1, ew_bytes[L][k], ew_macs[L][k], tag);
-          ctx->stats.launches++;
-        }
-      } else if (e == hipSuccess && lo[L].ew_total > 0) {
-        if (prof) ev = ctx->prof.begin(ctx->stream);
-        e = launch_ew(dtype, (const EwDesc*)(dbase + lo[L].ew), (const int64_t*)(dbase + lo[L].ewp),
-                      (int)ew[L].size(), lo[L].ew_total,
-                      ew_grid(ctx, lo[L].ew_total, ew_chunk(ctx, lo[L].ew_total)),
-                      ew_chunk(ctx, lo[L].ew_total), ctx->stream);
-        if (prof)
-          ctx->prof.end(ctx->stream, ev, 1, L, (int)ew[L].size(), lvl_bytes[1][L], lvl_macs[1][L]);
-        ctx->stats.launches++;
-      }
-      if (e == hipSuccess && lo[L].gm_total > 0 && per_step) {
-        for (size_t k = 0; k < gm[L].size() && e == hipSuccess; k++) {
-          const GemmDesc& d = gm[L][k];
-          ev = ctx->prof.begin(ctx->stream);
-          e = launch_gemm(dtype, true, (const GemmDesc*)(dbase + lo[L].gm) + k,
-                          (const int64_t*)(dbase + lo[L].onegm) + 2 * k, 1, gemm_tiles(d),
-                          gemm_grid(ctx, gemm_tiles(d)), ctx->stream);
-          char tag[128];
-          std::snprintf(tag, sizeof tag, "M=%d N=%d K=%d batch=%d", d.M, d.N, d.K, d.batch);
-          ctx->prof.end(ctx->stream, ev, 2, L, 1, gm_bytes[L][k], gm_macs[L][k], tag);
-          ctx->stats.launches++;
-        }
-      } else if (e == hipSuccess && lo[L].gm_total > 0) {
-        if (prof) ev = ctx->prof.begin(ctx->stream);
-        e = launch_gemm(dtype, true, (const GemmDesc*)(dbase + lo[L].gm),
-                        (const int64_t*)(dbase + lo[L].gmp), (int)gm[L].size(), lo[L].gm_total,
-                        gemm_grid(ctx, lo[L].gm_total), ctx->stream);
-        if (prof)
-          ctx->prof.end(ctx->stream, ev, 2, L, (int)gm[L].size(), lvl_bytes[2][L], lvl_macs[2][L]);
+        e = launch(k, dbase + o_desc[w], reinterpret_cast<const int64_t*>(dbase + o_pfx[w]),
+                   cnt[w], total);
+        if (prof) ctx->prof.end(ctx->stream, ev, k, L, cnt[w], lvl_bytes[w], lvl_macs[w]);
         ctx->stats.launches++;
       }
       ctx->stats.waves++;
     }
     if (e == hipSuccess) {
-      e = launch_epilogue(dtype, (const EpiDesc*)(dbase + o_epi), (int64_t)epi.size(),
-                          dbase + o_res, ctx->stream);
+      e = launch_epilogue(dtype, (const EpiDesc*)(dbase + o_epi), (int64_t)nb, dbase + o_res,
+                          ctx->stream);
       ctx->stats.launches++;
     }
     if (e == hipSuccess)
-      e = hipMemcpyAsync(ctx->staging.host, dbase + o_res, 4 * (i1 - i0), hipMemcpyDeviceToHost,
-                         ctx->stream);
+      e = hipMemcpyAsync(hb, dbase + o_res, 4 * nb, hipMemcpyDeviceToHost, ctx->stream);
+    ctx->stats.host_ms[2] += ms_since(t_issue);
+    auto t_wait = std::chrono::steady_clock::now();
     hipError_t es = hipStreamSynchronize(ctx->stream);
+    ctx->stats.host_ms[3] += ms_since(t_wait);
     if (prof) ctx->prof.collect(ctx->stats);
     if (check(e, "batched launch") || check(es, "batched sync")) {
       cleanup();
       return TBH_EDEVICE;
     }
-    std::memcpy(results + i0, ctx->staging.host, 4 * (i1 - i0));
+    std::memcpy(results + i0, hb, 4 * nb);
     cleanup();
     i0 = i1;
   }
@@ -804,6 +807,7 @@ static int contract_impl(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
   }
   std::lock_guard<std::mutex> g(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
+  auto t_plan = std::chrono::steady_clock::now();
   std::vector<SliceOwned> owned(n);
   std::vector<const SliceOwned*> ptrs(n);
   for (size_t i = 0; i < n; i++) {
@@ -816,6 +820,7 @@ static int contract_impl(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
   }
   std::vector<SlicePlan> plans;
   int rc = plan_all(ctx, ptrs, mode == TBH_BATCHED, &plans);
+  ctx->stats.host_ms[0] += ms_since(t_plan);
   if (rc) return rc;
   uint32_t* out = (uint32_t*)results;
   if (mode == TBH_BATCHED) {

@@ -163,7 +163,9 @@ typedef struct {
   int32_t tile_bits;        /* EW: outputs per tile (log2)                  */
   int32_t outer_reduced;    /* EW: reduced labels looped outside LDS        */
   int32_t direct;           /* EW: 1 = direct (wave) step, 0 = LDS-tiled    */
-  int32_t pad_;
+  int32_t tile_a_only;      /* EW: tile labels of A and not B               */
+  int32_t tile_b_only;      /* EW: tile labels of B and not A               */
+  int32_t outer_product;    /* EW: 1 = 4 x 4 outer-product tile layout      */
   double macs, bytes;       /* executed configurations, algorithmic bytes   */
 } tbh_step_info;
 /* Plans one slice and writes up to cap step records; returns the number of

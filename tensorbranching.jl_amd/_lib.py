@@ -44,7 +44,8 @@ class TbhStepInfo(C.Structure):
                 ("rank_b", C.c_int32), ("rank_c", C.c_int32), ("n_reduced", C.c_int32),
                 ("M", C.c_int32), ("N", C.c_int32), ("K", C.c_int32), ("batch", C.c_int32),
                 ("lanes_a", C.c_int32), ("lanes_b", C.c_int32), ("tile_bits", C.c_int32),
-                ("outer_reduced", C.c_int32), ("direct", C.c_int32), ("pad_", C.c_int32),
+                ("outer_reduced", C.c_int32), ("direct", C.c_int32), ("tile_a_only", C.c_int32),
+                ("tile_b_only", C.c_int32), ("outer_product", C.c_int32),
                 ("macs", C.c_double), ("bytes", C.c_double)]
 
 

@@ -231,7 +231,10 @@ __device__ __forceinline__ void ew_stage(const TBH_GLOBAL T* __restrict__ X, int
 // lds_pad(x) + lds_pad(y) when x and y share no bit, so the padded address of
 // (output m, reduced configuration) is a per-thread part plus a wave-uniform
 // part: one add per LDS read.
-template <typename T, bool HASB, int CNT>
+// OP (CNT = 16, binary, descriptor mlay = 1): tile bits 8, 9 are A-only and
+// 10, 11 B-only labels, so output m = ma + 4 mb needs A value ma and B value
+// mb: 4 + 4 LDS reads feed 16 maxima per reduced configuration.
+template <typename T, bool HASB, int CNT, bool OP = false>
 __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
                                         const EwThread& e, T* __restrict__ lA,
                                         T* __restrict__ lB) {
@@ -274,7 +277,64 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
     ew_stage<T>(A + cA, d->nSA, e.loSA, e.hSA, lA);
     if (HASB) ew_stage<T>(B + cB, d->nSB, e.loSB, e.hSB, lB);
     __syncthreads();
-    if (active) {
+    if (OP && active) {
+      uint32_t ltA = e.loTA, ltB = e.loTB;
+      asm volatile("" : "+v"(ltA), "+v"(ltB));
+      uint32_t pA[4], pB[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        pA[j] = lds_pad(ltA + hi_at(e.hTA, j));
+        pB[j] = lds_pad(ltB + hi_at(e.hTB, 4 * j));
+      }
+      uint32_t rA = 0, rB = 0;
+      for (uint32_t h = 0; h < nh; h++) {
+        if (h) {
+          const int i = __builtin_ctz(h);
+          const uint32_t g = h ^ (h >> 1);
+          const uint32_t sa = pos_stride(TBH_KF(d, rA)[nlo + i]), sb = pos_stride(TBH_KF(d, rB)[nlo + i]);
+          if ((g >> i) & 1) {
+            rA += sa;
+            rB += sb;
+          } else {
+            rA -= sa;
+            rB -= sb;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+          if (q >= nq) break;
+          const uint32_t qa0 = lds_pad(rA + hi_at(lsA, q)), qb0 = lds_pad(rB + hi_at(lsB, q));
+          T a0[4], b0[4];
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            a0[j] = lA[pA[j] + qa0];
+            b0[j] = lB[pB[j] + qb0];
+          }
+          if (nq > 1) {
+            const uint32_t qa1 = lds_pad(rA + hi_at(lsA, q + 1));
+            const uint32_t qb1 = lds_pad(rB + hi_at(lsB, q + 1));
+            T a1[4], b1[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              a1[j] = lA[pA[j] + qa1];
+              b1[j] = lB[pB[j] + qb1];
+            }
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+              for (int ma = 0; ma < 4; ma++)
+                acc[ma + 4 * mb] = max3(acc[ma + 4 * mb], Trop<T>::mul(a0[ma], b0[mb]),
+                                        Trop<T>::mul(a1[ma], b1[mb]));
+          } else {
+#pragma unroll
+            for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+              for (int ma = 0; ma < 4; ma++)
+                acc[ma + 4 * mb] = Trop<T>::add(acc[ma + 4 * mb], Trop<T>::mul(a0[ma], b0[mb]));
+          }
+        }
+      }
+    } else if (active) {
       // per-thread LDS addresses, recomputed per chunk: the empty asm hides
       // that they are loop-invariant, so they are not held in registers
       // across the staging
@@ -314,7 +374,7 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
                 x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
                 x1 = Trop<T>::mul(x1, lB[pB[m] + qb1]);
               }
-              acc[m] = Trop<T>::add(acc[m], Trop<T>::add(x0, x1));
+              acc[m] = max3(acc[m], x0, x1);
             }
           } else {
 #pragma unroll
@@ -347,7 +407,12 @@ __device__ __forceinline__ void ew_tile_dispatch(KEw d, const EwOuter& off,
     case 1: ew_tile<T, HASB, 2>(d, off, e, lA, lB); break;
     case 2: ew_tile<T, HASB, 4>(d, off, e, lA, lB); break;
     case 3: ew_tile<T, HASB, 8>(d, off, e, lA, lB); break;
-    default: ew_tile<T, HASB, 16>(d, off, e, lA, lB); break;
+    default:
+      if (HASB && d->mlay == 1)
+        ew_tile<T, HASB, 16, HASB>(d, off, e, lA, lB);
+      else
+        ew_tile<T, HASB, 16>(d, off, e, lA, lB);
+      break;
   }
 }
 

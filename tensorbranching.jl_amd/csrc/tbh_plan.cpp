@@ -252,6 +252,30 @@ struct Builder {
     }
     std::sort(T.begin(), T.end(),
               [&](int32_t x, int32_t y) { return pos_in(LC, x) < pos_in(LC, y); });
+    // a full binary tile: tile bits 0..5 (the lanes) stay C's lowest tile
+    // positions (coalesced writes); bits 8, 9 become A-only and 10, 11
+    // B-only labels when the tile has them, so each thread's 16 outputs are
+    // a 4 x 4 outer product of 4 A and 4 B values per reduced configuration
+    int mlay = 0;
+    if (!op.direct && b >= 0 && (int)T.size() == kTileBits) {
+      std::vector<int32_t> ao, bo, rest;
+      for (int k = 6; k < kTileBits; k++) {
+        const int32_t l = T[k];
+        if (inA(l) && !inB(l) && ao.size() < 2)
+          ao.push_back(l);
+        else if (inB(l) && !inA(l) && bo.size() < 2)
+          bo.push_back(l);
+        else
+          rest.push_back(l);
+      }
+      if (ao.size() == 2 && bo.size() == 2) {
+        T.resize(6);
+        T.insert(T.end(), rest.begin(), rest.end());
+        T.insert(T.end(), ao.begin(), ao.end());
+        T.insert(T.end(), bo.begin(), bo.end());
+        mlay = 1;
+      }
+    }
     std::vector<int32_t> O;
     for (int32_t l : LC)
       if (std::find(T.begin(), T.end(), l) == T.end()) O.push_back(l);
@@ -268,6 +292,7 @@ struct Builder {
     d.nO = (int)O.size();
     d.nR = (int)Ri.size();
     d.nRo = (int)Ro.size();
+    d.mlay = mlay;
     d.nSA = (int)SA.size();
     d.nSB = (int)SB.size();
     d.tiles = 1 << d.nO;
@@ -419,6 +444,11 @@ int verify_plan(const SlicePlan& P, std::string* err) {
           return bad("EW tile bits");
       for (int k = d.nT; k < kTileBits; k++)
         if (d.tC[k] != -1 || d.tA[k] != -1 || d.tB[k] != -1) return bad("EW tile padding");
+      if (d.mlay != 0 &&
+          (d.mlay != 1 || op.direct || op.b < 0 || d.nT != kTileBits || d.tA[8] < 0 ||
+           d.tA[9] < 0 || d.tB[8] >= 0 || d.tB[9] >= 0 || d.tB[10] < 0 || d.tB[11] < 0 ||
+           d.tA[10] >= 0 || d.tA[11] >= 0))
+        return bad("EW outer-product tile layout");
       for (int i = 0; i < kMaxReduce; i++) {
         if (i >= d.nR) {
           if (d.rA[i] != -1 || d.rB[i] != -1) return bad("EW reduced padding");

@@ -1092,6 +1092,11 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode, tbh_s
       s.tile_bits = op.ew.nT;
       s.outer_reduced = op.ew.nRo;
       s.direct = op.direct ? 1 : 0;
+      s.outer_product = op.ew.mlay;
+      for (int k = 0; k < op.ew.nT; k++) {
+        s.tile_a_only += op.ew.tA[k] >= 0 && op.ew.tB[k] < 0;
+        s.tile_b_only += op.ew.tB[k] >= 0 && op.ew.tA[k] < 0;
+      }
     }
     s.macs = op.macs;
     s.bytes = op.bytes;

@@ -111,3 +111,23 @@ def test_slicing_invariant(tb, orc, backend):
     whole = tb.SlicedBranch(tb.MISProblem(g, w), tb.initialize_code(g, 20), 0)
     assert backend.contract_slices(br).max() == backend.solve_slice(whole) == \
         orc.solve_slice(whole)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.int32])
+def test_tile_variants_bit_exact(tb, orc, backend, dtype):
+    """A workload whose batched plan has 4 x 4 outer-product tiles, tiles
+    with outer reduced labels (looped outside LDS), GEMM, direct and tiled
+    steps: every slice's optimum equals the oracle's in both modes."""
+    g = tb.random_regular_graph(160, 3, seed=5)
+    w = None
+    if dtype == np.int32:
+        w = np.random.default_rng(5).integers(1, 101, g.nv).astype(np.int32)
+    br = _slices(tb, g, 18, weights=w, seed=5, trials=30)
+    steps = [s for b in br if b.code is not None for s in tb.plan_steps(b, dtype)]
+    ew = [s for s in steps if s["kind"] == 0]
+    assert any(s["outer_product"] for s in ew)
+    assert any(s["outer_reduced"] > 0 for s in ew)
+    assert any(s["direct"] for s in ew) and any(not s["direct"] for s in ew)
+    ref = orc.contract_slices(br, dtype)
+    for mode in ("batched", "plain"):
+        assert np.array_equal(backend.contract_slices(br, dtype, mode), ref), mode

@@ -699,6 +699,10 @@ int plan_slice(const SliceIn& s_in, const PlanOptions& opt_in, SlicePlan* out, s
   };
 
   Builder B{s, opt, P, {}, err};
+  // a node becomes up to three steps (permutes, split-K reductions)
+  P.ops.reserve(3 * (size_t)nn + 4);
+  P.tensors.reserve(3 * (size_t)(nl + nn) + 4);
+  B.layout.reserve(3 * (size_t)(nl + nn) + 4);
   // leaves are tensors 0..nl-1 (content is symmetric, so any layout works)
   for (int32_t i = 0; i < nl; i++) {
     B.add_tensor(opt.batched ? sorted_key(lab[i]) : lab[i]);

@@ -258,6 +258,53 @@ int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool ba
   return 0;
 }
 
+PlanStream::PlanStream(const std::vector<const SliceOwned*>& slices, bool batched, int threads)
+    : slices_(slices), plans_(slices.size()), done_(slices.size(), 0) {
+  PlanOptions opt;
+  opt.batched = batched;
+  auto worker = [this, opt]() {
+    for (;;) {
+      const size_t i = next_.fetch_add(1);
+      if (i >= slices_.size()) return;
+      {
+        std::lock_guard<std::mutex> g(m_);
+        if (i > bad_) return;
+      }
+      std::string err;
+      const int rc = plan_slice(slices_[i]->in, opt, &plans_[i], &err);
+      {
+        std::lock_guard<std::mutex> g(m_);
+        if (rc && i < bad_) {
+          bad_ = i;
+          bad_err_ = "slice " + std::to_string(i) + ": " + err;
+        }
+        done_[i] = 1;
+      }
+      cv_.notify_all();
+    }
+  };
+  const int nt = (int)std::min<size_t>(std::max(1, threads), std::max<size_t>(1, slices.size()));
+  for (int k = 0; k < nt; k++) th_.emplace_back(worker);
+}
+
+PlanStream::~PlanStream() {
+  {
+    std::lock_guard<std::mutex> g(m_);
+    bad_ = 0;  // stop the workers after their current slice
+  }
+  for (auto& t : th_) t.join();
+}
+
+const SlicePlan* PlanStream::get(size_t i, std::string* err) {
+  std::unique_lock<std::mutex> g(m_);
+  cv_.wait(g, [&] { return done_[i] || i >= bad_; });
+  if (i >= bad_) {
+    if (err) *err = bad_err_;
+    return nullptr;
+  }
+  return &plans_[i];
+}
+
 // ---------------------------------------------------- descriptor packs ---
 namespace {
 
@@ -366,53 +413,105 @@ void parallel_for(size_t n, int threads, F f) {
 
 }  // namespace
 
-// A batch is packed in two passes.  Pass 1 (serial, integer work only)
-// counts the steps of every (kernel kind, level), gives each slice its first
-// index in each, and builds the tile prefixes; pass 2 (host threads, one
-// slice at a time) writes every descriptor straight into its final place in
-// the pinned staging buffer, which goes to the device in one copy.
+// Batches run one after another on the stream through one arena region
+// (stream order keeps a batch's kernels behind the previous batch's).  A
+// batch is packed in two passes on the host while the previous batch runs:
+// pass 1 (serial, integer work only) counts the steps of every (kernel kind,
+// level), gives each slice its first index in each and builds the tile
+// prefixes; pass 2 (host threads, one slice at a time) writes every
+// descriptor straight into its final place in a pinned staging buffer (two,
+// used alternately), which goes to the device in one copy.  The previous
+// batch is waited for only after the next one is enqueued.
 int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*>& slices,
-                    const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* results) {
+                    const PlanOf& plan_of, bool add_r, uint32_t* results) {
   const size_t n = slices.size();
+  if (n == 0) return 0;
   const uint32_t zero_bits = dtype == TBH_F32 ? f2b(0.0f) : 0u;
   const bool per_step = ctx->profiling && std::getenv("TBH_PROFILE_STEPS");
+  const bool prof = ctx->profiling;
+  int rc = ctx->results_host.ensure(4 * n);
+  if (rc) return rc;
+  uint32_t* res_host = static_cast<uint32_t*>(ctx->results_host.host);
+  // the arena region: the largest free block less a reserve for descriptors
+  const size_t largest = ctx->pool.largest_free();
+  const size_t reserve = std::min<size_t>(
+      (size_t)8 << 30, std::max(largest / 8, std::min<size_t>(largest / 4, (size_t)16 << 20)));
+  const size_t region = largest - reserve;
+  const int64_t region_off = region >= 256 ? ctx->pool.alloc(region) : -1;
+  if (region_off < 0) {
+    set_error("pool too small for a slice arena");
+    return TBH_ENOMEM;
+  }
+  const uint64_t arena_base = (uint64_t)(ctx->base + region_off);
+  struct Pending {
+    bool on = false;
+    hipEvent_t done = nullptr;
+    int64_t doff = -1;
+  } prev;
+  hipEvent_t evs[2] = {nullptr, nullptr};
+  for (auto& ev : evs)
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+  std::string err;
+  int status = 0;
+  auto fail = [&](int code, const std::string& m) {
+    set_error(m);
+    status = code;
+  };
+  // waits for the batch in flight and releases its descriptors
+  auto finish = [&](Pending& p) {
+    if (!p.on) return true;
+    auto t_wait = std::chrono::steady_clock::now();
+    hipError_t e = p.done ? hipEventSynchronize(p.done) : hipStreamSynchronize(ctx->stream);
+    ctx->stats.host_ms[3] += ms_since(t_wait);
+    ctx->pool.free(p.doff);
+    p.on = false;
+    if (e != hipSuccess) {
+      fail(TBH_EDEVICE, std::string("batched execution: ") + hipGetErrorString(e));
+      return false;
+    }
+    return true;
+  };
+  // the first batch is kept to a fraction of the region so that the GPU
+  // starts while the later slices are still being planned
+  static const double first_frac = [] {
+    const char* e = std::getenv("TBH_FIRST_BATCH");  // tuning sweeps
+    return e ? std::min(1.0, std::max(0.01, std::atof(e))) : 0.15;
+  }();
   size_t i0 = 0;
-  while (i0 < n) {
-    auto t_pack = std::chrono::steady_clock::now();
-    // admit slices while their arenas fit (descriptors are counted per
-    // slice; the reserve covers the pool's first-fit fragmentation: 1/16 of
-    // the free block, <= 64 MiB)
-    size_t budget = ctx->pool.largest_free();
-    budget -= std::min<size_t>((size_t)64 << 20, budget / 16);
-    size_t i1 = i0, arena = 0, ndesc_bytes = 0;
+  for (int bidx = 0; i0 < n && status == 0; bidx++) {
+    auto t_plan = std::chrono::steady_clock::now();
+    // admit slices while their arenas fit the region (waiting for plans)
+    const size_t cap = bidx == 0 ? (size_t)(first_frac * (double)region) : region;
+    std::vector<const SlicePlan*> plans;
+    size_t i1 = i0, arena = 0;
     while (i1 < n) {
-      const SlicePlan& P = *plans[i1];
-      size_t a = P.empty ? 0 : (size_t)P.arena_bytes;
-      size_t d = P.ops.size() * (sizeof(EwDesc) + 16) + P.leaves.size() * sizeof(LeafDesc) + 512;
-      if (i1 > i0 && arena + a + 2 * (ndesc_bytes + d) > budget) break;
+      const SlicePlan* P = plan_of(i1, &err);
+      if (!P) {
+        fail(TBH_EINVAL, err);
+        break;
+      }
+      const size_t a = P->empty ? 0 : (size_t)P->arena_bytes;
+      if (i1 > i0 && arena + a > cap) break;
+      if (a > region) {
+        fail(TBH_ENOMEM, "slice arena (" + std::to_string(a) + " bytes) does not fit the pool");
+        break;
+      }
       arena += a;
-      ndesc_bytes += d;
+      plans.push_back(P);
       i1++;
     }
-    if (arena + 2 * ndesc_bytes > budget) {
-      set_error("slice arena (" + std::to_string(arena) + " bytes) does not fit the pool");
-      return TBH_ENOMEM;
-    }
+    ctx->stats.host_ms[0] += ms_since(t_plan);
+    if (status) break;
+    auto t_pack = std::chrono::steady_clock::now();
     const size_t nb = i1 - i0;
-    int64_t arena_off = ctx->pool.alloc(std::max<size_t>(arena, 256));
-    if (arena_off < 0) {
-      set_error("pool exhausted");
-      return TBH_ENOMEM;
-    }
-    const uint64_t arena_base = (uint64_t)(ctx->base + arena_off);
     std::vector<uint64_t> sbase(nb);
     int H = 0;
     {
       uint64_t acc = 0;
       for (size_t i = 0; i < nb; i++) {
         sbase[i] = arena_base + acc;
-        acc += plans[i0 + i]->empty ? 0 : (uint64_t)plans[i0 + i]->arena_bytes;
-        H = std::max(H, plans[i0 + i]->nlevels);
+        acc += plans[i]->empty ? 0 : (uint64_t)plans[i]->arena_bytes;
+        H = std::max(H, plans[i]->nlevels);
       }
     }
     const int W = 3 * (H + 1);  // (kind, level) slots
@@ -424,7 +523,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     std::vector<std::vector<std::pair<double, double>>> step_work(per_step ? W : 0);
     std::vector<int64_t> leaf_start(nb + 1, 0);
     for (size_t i = 0; i < nb; i++) {
-      const SlicePlan& P = *plans[i0 + i];
+      const SlicePlan& P = *plans[i];
       std::memcpy(&start[i * W], cnt.data(), W * sizeof(int32_t));
       leaf_start[i + 1] = leaf_start[i] + (P.empty ? 0 : (int64_t)P.leaves.size());
       if (P.empty) continue;
@@ -465,20 +564,12 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
       if (per_step) o_one[w] = take((size_t)cnt[w] * 2 * sizeof(int64_t));
     }
     const size_t bytes = off;
-    int64_t doff = ctx->pool.alloc(bytes);
-    if (doff < 0) {
-      ctx->pool.free(arena_off);
-      set_error("pool exhausted (descriptors)");
-      return TBH_ENOMEM;
+    Staging& stg = (bidx & 1) ? ctx->staging2 : ctx->staging;
+    if ((rc = stg.ensure(bytes))) {
+      status = rc;
+      break;
     }
-    char* dbase = ctx->base + doff;
-    int rc = ctx->staging.ensure(bytes);
-    if (rc) {
-      ctx->pool.free(doff);
-      ctx->pool.free(arena_off);
-      return rc;
-    }
-    char* hb = static_cast<char*>(ctx->staging.host);
+    char* hb = static_cast<char*>(stg.host);
     for (int w = 0; w < W; w++) {
       if (!cnt[w]) continue;
       std::memcpy(hb + o_pfx[w], pfx[w].data(), pfx[w].size() * sizeof(int64_t));
@@ -494,7 +585,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     std::vector<std::vector<int32_t>> local(std::max(1, ctx->plan_threads),
                                             std::vector<int32_t>(W));
     parallel_for(nb, ctx->plan_threads, [&](size_t i, int t) {
-      const SlicePlan& P = *plans[i0 + i];
+      const SlicePlan& P = *plans[i];
       const uint64_t b = sbase[i];
       EpiDesc e;
       std::memset(&e, 0, sizeof e);
@@ -522,26 +613,23 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
       }
     });
     ctx->stats.host_ms[1] += ms_since(t_pack);
-    auto t_issue = std::chrono::steady_clock::now();
-    auto cleanup = [&]() {
-      ctx->pool.free(doff);
-      ctx->pool.free(arena_off);
-    };
-    auto check = [&](hipError_t e, const char* what) {
-      if (e != hipSuccess) {
-        set_error(std::string(what) + ": " + hipGetErrorString(e));
-        return true;
-      }
-      return false;
-    };
-    if (check(hipMemcpyAsync(dbase, hb, bytes, hipMemcpyHostToDevice, ctx->stream), "upload")) {
-      (void)hipStreamSynchronize(ctx->stream);
-      cleanup();
-      return TBH_EDEVICE;
+    // device descriptors: if the reserve cannot hold two batches' worth,
+    // wait for the batch in flight first
+    int64_t doff = ctx->pool.alloc(bytes);
+    if (doff < 0) {
+      if (!finish(prev)) break;
+      doff = ctx->pool.alloc(bytes);
     }
-    hipError_t e = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), nleaves, ctx->stream);
+    if (doff < 0) {
+      fail(TBH_ENOMEM, "pool exhausted (descriptors)");
+      break;
+    }
+    auto t_issue = std::chrono::steady_clock::now();
+    char* dbase = ctx->base + doff;
+    hipError_t e = hipMemcpyAsync(dbase, hb, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), nleaves, ctx->stream);
     ctx->stats.launches += nleaves ? 1 : 0;
-    const bool prof = ctx->profiling;
     hipEvent_t ev = nullptr;
     // one launch per kernel kind and level (direct EW, tiled EW, GEMM); in
     // the per-step profiling mode every step alone, tagged with its shape
@@ -571,8 +659,8 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
             } else {
               const EwDesc& d = reinterpret_cast<const EwDesc*>(hb + o_desc[w])[q];
               if (k == 0 && d.nO < 8) continue;  // only the big direct steps
-              std::snprintf(tag, sizeof tag, "nT=%d nO=%d nR=%d nRo=%d nSA=%d nSB=%d", d.nT, d.nO,
-                            d.nR, d.nRo, d.nSA, d.nSB);
+              std::snprintf(tag, sizeof tag, "nT=%d nO=%d nR=%d nRo=%d nSA=%d nSB=%d op=%d", d.nT,
+                            d.nO, d.nR, d.nRo, d.nSA, d.nSB, d.mlay);
             }
             const int64_t tiles = pfx[w][q + 1] - pfx[w][q];
             ev = ctx->prof.begin(ctx->stream);
@@ -598,20 +686,35 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
       ctx->stats.launches++;
     }
     if (e == hipSuccess)
-      e = hipMemcpyAsync(hb, dbase + o_res, 4 * nb, hipMemcpyDeviceToHost, ctx->stream);
+      e = hipMemcpyAsync(res_host + i0, dbase + o_res, 4 * nb, hipMemcpyDeviceToHost, ctx->stream);
+    hipEvent_t done = evs[bidx & 1];
+    if (e == hipSuccess && done) e = hipEventRecord(done, ctx->stream);
     ctx->stats.host_ms[2] += ms_since(t_issue);
-    auto t_wait = std::chrono::steady_clock::now();
-    hipError_t es = hipStreamSynchronize(ctx->stream);
-    ctx->stats.host_ms[3] += ms_since(t_wait);
-    if (prof) ctx->prof.collect(ctx->stats);
-    if (check(e, "batched launch") || check(es, "batched sync")) {
-      cleanup();
-      return TBH_EDEVICE;
+    if (e != hipSuccess) {
+      (void)hipStreamSynchronize(ctx->stream);
+      ctx->pool.free(doff);
+      fail(TBH_EDEVICE, std::string("batched launch: ") + hipGetErrorString(e));
+      break;
     }
-    std::memcpy(results + i0, hb, 4 * nb);
-    cleanup();
+    // the next batch is packed while this one runs; the previous is done
+    if (!finish(prev)) {
+      (void)hipStreamSynchronize(ctx->stream);
+      ctx->pool.free(doff);
+      break;
+    }
+    prev.on = true;
+    prev.done = done;
+    prev.doff = doff;
     i0 = i1;
   }
+  if (prev.on && !finish(prev) && status == 0) status = TBH_EDEVICE;
+  if (status) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& ev : evs)
+    if (ev) (void)hipEventDestroy(ev);
+  ctx->pool.free(region_off);
+  if (prof) ctx->prof.collect(ctx->stats);
+  if (status) return status;
+  std::memcpy(results, res_host, 4 * n);
   return 0;
 }
 
@@ -818,16 +921,18 @@ static int contract_impl(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
     }
     ptrs[i] = &owned[i];
   }
-  std::vector<SlicePlan> plans;
-  int rc = plan_all(ctx, ptrs, mode == TBH_BATCHED, &plans);
-  ctx->stats.host_ms[0] += ms_since(t_plan);
-  if (rc) return rc;
   uint32_t* out = (uint32_t*)results;
   if (mode == TBH_BATCHED) {
-    std::vector<const SlicePlan*> pp(n);
-    for (size_t i = 0; i < n; i++) pp[i] = &plans[i];
-    return execute_batched(ctx, dtype, ptrs, pp, add_r, out);
+    // slices are planned on host threads while earlier batches run
+    PlanStream ps(ptrs, true, ctx->plan_threads);
+    ctx->stats.host_ms[0] += ms_since(t_plan);
+    return execute_batched(ctx, dtype, ptrs,
+                           [&](size_t i, std::string* err) { return ps.get(i, err); }, add_r, out);
   }
+  std::vector<SlicePlan> plans;
+  int rc = plan_all(ctx, ptrs, false, &plans);
+  ctx->stats.host_ms[0] += ms_since(t_plan);
+  if (rc) return rc;
   for (size_t i = 0; i < n; i++) {
     rc = execute_plain(ctx, dtype, owned[i], plans[i], add_r, out + i);
     if (rc) return rc;

@@ -5,6 +5,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -88,7 +89,8 @@ struct tbh_ctx {
   hipStream_t stream = nullptr;
   char* base = nullptr;  // device pool
   tbh::Pool pool;
-  tbh::Staging staging;
+  tbh::Staging staging, staging2;  // descriptor uploads (ping-pong across batches)
+  tbh::Staging results_host;       // per-slice results read back
   std::mutex mu;         // one executor call at a time
   tbh_stats stats{};
   int plan_threads = 8;
@@ -111,10 +113,33 @@ struct SliceOwned {
 
 int convert_slice(const tbh_slice& s, tbh_dtype dtype, bool copy, SliceOwned* out);
 
-// Executes plans for `slices`; results[i] (dtype bits) = root + r (add_r) or
-// root.  Used by the C ABI and by the serving scheduler.
+// Plans slices on background host threads in index order; get(i) blocks
+// until slice i is planned and returns nullptr (error in *err) if planning
+// failed at or before i.
+class PlanStream {
+ public:
+  PlanStream(const std::vector<const SliceOwned*>& slices, bool batched, int threads);
+  ~PlanStream();
+  const SlicePlan* get(size_t i, std::string* err);
+
+ private:
+  const std::vector<const SliceOwned*>& slices_;
+  std::vector<SlicePlan> plans_;
+  std::vector<char> done_;
+  std::mutex m_;
+  std::condition_variable cv_;
+  std::atomic<size_t> next_{0};
+  size_t bad_ = (size_t)-1;
+  std::string bad_err_;
+  std::vector<std::thread> th_;
+};
+
+// Executes the plans of `slices` (plan_of(i) may block until slice i is
+// planned, nullptr = planning failed); results[i] (dtype bits) = root + r
+// (add_r) or root.  Used by the C ABI and by the serving scheduler.
+using PlanOf = std::function<const SlicePlan*(size_t, std::string*)>;
 int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*>& slices,
-                    const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* results);
+                    const PlanOf& plan_of, bool add_r, uint32_t* results);
 int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SlicePlan& plan,
                   bool add_r, uint32_t* result);
 int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool batched,

@@ -147,7 +147,8 @@ struct ServeState {
       int rc;
       {
         std::lock_guard<std::mutex> g(ctx->mu);
-        rc = execute_batched(ctx, dtype, sl, pl, true, res.data());
+        rc = execute_batched(ctx, dtype, sl, [&](size_t i, std::string*) { return pl[i]; }, true,
+                             res.data());
       }
       const std::string err = rc ? std::string(tbh_last_error()) : std::string();
       // fold results

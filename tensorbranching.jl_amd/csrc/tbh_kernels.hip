@@ -156,6 +156,29 @@ __device__ __forceinline__ int64_t uniform64(int64_t x) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// The step holding tile t, searched forward from step op (prefix[op] <= t;
+// a wave's or workgroup's chunks only move forward): galloping, then
+// bisection, so a search that stays in or near the current step reads one or
+// two prefix entries instead of log2(nops).  prefix[nops] = total > t.
+__device__ __forceinline__ int find_step(const int64_t* __restrict__ prefix, int nops, int op,
+                                         int64_t t) {
+  int lo = op, hi = op + 1, step = 1;
+  while (hi < nops && prefix[hi] <= t) {
+    lo = hi;
+    step <<= 1;
+    hi = lo + step;
+  }
+  if (hi > nops) hi = nops;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
 // outer offsets of tile o (sum of the strides of its set bits)
 struct EwOuter {
   uint32_t a, b, c;
@@ -536,21 +559,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBH_DIRECT_
   const int64_t gw = (int64_t)blockIdx.x * 4 + wid;
   EwLane L;
   EwOuter off{0, 0, 0};
-  int op = -1;
-  int64_t ob = 0, oe = 0;
+  int op = 0;
+  int64_t ob = 0, oe = 0;  // oe = 0: the first chunk searches
   for (int64_t c0 = gw * chunk; c0 < total; c0 += nw * chunk) {
     const int64_t c1 = c0 + chunk < total ? c0 + chunk : total;
-    // one search per chunk; inside it the tiles (and steps) only move forward
-    {
-      int lo = 0, hi = nops;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= c0)
-          lo = mid;
-        else
-          hi = mid;
-      }
-      op = __builtin_amdgcn_readfirstlane(lo);
+    // a wave's chunks only move forward: search from the current step
+    if (c0 >= oe) {
+      op = find_step(prefix, nops, op, c0);
       ob = uniform64(((const TBH_KCONST int64_t*)prefix)[op]);
       oe = uniform64(((const TBH_KCONST int64_t*)prefix)[op + 1]);
       ew_lane_setup((KEw)(ops + op), lane, L);
@@ -587,21 +602,13 @@ __global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW
   __shared__ T lA[kLdsElems];
   __shared__ T lB[kLdsElems];
   EwThread e;
-  int op = -1;
-  int64_t ob = 0, oe = 0;
+  int op = 0;
+  int64_t ob = 0, oe = 0;  // oe = 0: the first chunk searches
   for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < total; c0 += (int64_t)gridDim.x * chunk) {
     const int64_t c1 = c0 + chunk < total ? c0 + chunk : total;
-    // one search per chunk; inside it the tiles (and steps) only move forward
-    {
-      int lo = 0, hi = nops;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= c0)
-          lo = mid;
-        else
-          hi = mid;
-      }
-      op = __builtin_amdgcn_readfirstlane(lo);
+    // a workgroup's chunks only move forward: search from the current step
+    if (c0 >= oe) {
+      op = find_step(prefix, nops, op, c0);
       ob = uniform64(((const TBH_KCONST int64_t*)prefix)[op]);
       oe = uniform64(((const TBH_KCONST int64_t*)prefix)[op + 1]);
       ew_thread_setup((KEw)(ops + op), e);

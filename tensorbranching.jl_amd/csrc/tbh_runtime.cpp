@@ -380,7 +380,13 @@ int direct_chunk(tbh_ctx*, int64_t) {
   }();
   return c;
 }
-int ew_chunk(tbh_ctx*, int64_t) { return 2; }
+int ew_chunk(tbh_ctx*, int64_t) {
+  static const int c = [] {
+    const char* e = std::getenv("TBH_EW_CHUNK");  // tuning sweeps
+    return e ? std::max(1, std::atoi(e)) : 2;
+  }();
+  return c;
+}
 
 }  // namespace
 

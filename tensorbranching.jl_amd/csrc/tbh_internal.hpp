@@ -33,8 +33,9 @@ struct EwDesc {
   int32_t nSA, nSB;        // sub-tensor bits of A and B
   int32_t tiles;           // 2^nO
   int32_t nRo;             // outer reduced bits (tiled steps only)
-  int32_t mlay;            // 1: tile bits 8, 9 are A-only and 10, 11 B-only labels
-                           // (a thread's 16 outputs are a 4 x 4 outer product)
+  int32_t mlay;            // MA in 1..3: tile bits 8..8+MA-1 are A-only labels and
+                           // the rest of 8..11 B-only (a thread's 16 outputs are a
+                           // 2^MA x 2^(4-MA) outer product); 0: no such layout
   int8_t qA[kMaxReduce];   // outer reduced bit -> A / B position
   int8_t qB[kMaxReduce];
   int8_t tC[kTileBits];    // tile bit -> C position

@@ -254,10 +254,11 @@ __device__ __forceinline__ void ew_stage(const TBH_GLOBAL T* __restrict__ X, int
 // lds_pad(x) + lds_pad(y) when x and y share no bit, so the padded address of
 // (output m, reduced configuration) is a per-thread part plus a wave-uniform
 // part: one add per LDS read.
-// OP (CNT = 16, binary, descriptor mlay = 1): tile bits 8, 9 are A-only and
-// 10, 11 B-only labels, so output m = ma + 4 mb needs A value ma and B value
-// mb: 4 + 4 LDS reads feed 16 maxima per reduced configuration.
-template <typename T, bool HASB, int CNT, bool OP = false>
+// MA > 0 (CNT = 16, binary, descriptor mlay = MA): tile bits 8 .. 8+MA-1
+// are A-only and the rest of 8..11 B-only labels, so output m = ma + 2^MA mb
+// needs A value ma and B value mb: 2^MA + 2^(4-MA) LDS reads feed 16 maxima
+// per reduced configuration.
+template <typename T, bool HASB, int CNT, int MA = 0>
 __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
                                         const EwThread& e, T* __restrict__ lA,
                                         T* __restrict__ lB) {
@@ -300,15 +301,15 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
     ew_stage<T>(A + cA, d->nSA, e.loSA, e.hSA, lA);
     if (HASB) ew_stage<T>(B + cB, d->nSB, e.loSB, e.hSB, lB);
     __syncthreads();
-    if (OP && active) {
+    if (MA > 0 && active) {
+      constexpr int NA = 1 << MA, NB = 16 >> MA;
       uint32_t ltA = e.loTA, ltB = e.loTB;
       asm volatile("" : "+v"(ltA), "+v"(ltB));
-      uint32_t pA[4], pB[4];
+      uint32_t pA[NA], pB[NB];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        pA[j] = lds_pad(ltA + hi_at(e.hTA, j));
-        pB[j] = lds_pad(ltB + hi_at(e.hTB, 4 * j));
-      }
+      for (int j = 0; j < NA; j++) pA[j] = lds_pad(ltA + hi_at(e.hTA, j));
+#pragma unroll
+      for (int j = 0; j < NB; j++) pB[j] = lds_pad(ltB + hi_at(e.hTB, j << MA));
       uint32_t rA = 0, rB = 0;
       for (uint32_t h = 0; h < nh; h++) {
         if (h) {
@@ -327,33 +328,31 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
         for (int q = 0; q < 8; q += 2) {
           if (q >= nq) break;
           const uint32_t qa0 = lds_pad(rA + hi_at(lsA, q)), qb0 = lds_pad(rB + hi_at(lsB, q));
-          T a0[4], b0[4];
+          T a0[NA], b0[NB];
 #pragma unroll
-          for (int j = 0; j < 4; j++) {
-            a0[j] = lA[pA[j] + qa0];
-            b0[j] = lB[pB[j] + qb0];
-          }
+          for (int j = 0; j < NA; j++) a0[j] = lA[pA[j] + qa0];
+#pragma unroll
+          for (int j = 0; j < NB; j++) b0[j] = lB[pB[j] + qb0];
           if (nq > 1) {
             const uint32_t qa1 = lds_pad(rA + hi_at(lsA, q + 1));
             const uint32_t qb1 = lds_pad(rB + hi_at(lsB, q + 1));
-            T a1[4], b1[4];
+            T a1[NA], b1[NB];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-              a1[j] = lA[pA[j] + qa1];
-              b1[j] = lB[pB[j] + qb1];
-            }
+            for (int j = 0; j < NA; j++) a1[j] = lA[pA[j] + qa1];
 #pragma unroll
-            for (int mb = 0; mb < 4; mb++)
+            for (int j = 0; j < NB; j++) b1[j] = lB[pB[j] + qb1];
 #pragma unroll
-              for (int ma = 0; ma < 4; ma++)
-                acc[ma + 4 * mb] = max3(acc[ma + 4 * mb], Trop<T>::mul(a0[ma], b0[mb]),
-                                        Trop<T>::mul(a1[ma], b1[mb]));
+            for (int mb = 0; mb < NB; mb++)
+#pragma unroll
+              for (int ma = 0; ma < NA; ma++)
+                acc[ma + NA * mb] = max3(acc[ma + NA * mb], Trop<T>::mul(a0[ma], b0[mb]),
+                                         Trop<T>::mul(a1[ma], b1[mb]));
           } else {
 #pragma unroll
-            for (int mb = 0; mb < 4; mb++)
+            for (int mb = 0; mb < NB; mb++)
 #pragma unroll
-              for (int ma = 0; ma < 4; ma++)
-                acc[ma + 4 * mb] = Trop<T>::add(acc[ma + 4 * mb], Trop<T>::mul(a0[ma], b0[mb]));
+              for (int ma = 0; ma < NA; ma++)
+                acc[ma + NA * mb] = Trop<T>::add(acc[ma + NA * mb], Trop<T>::mul(a0[ma], b0[mb]));
           }
         }
       }
@@ -431,10 +430,15 @@ __device__ __forceinline__ void ew_tile_dispatch(KEw d, const EwOuter& off,
     case 2: ew_tile<T, HASB, 4>(d, off, e, lA, lB); break;
     case 3: ew_tile<T, HASB, 8>(d, off, e, lA, lB); break;
     default:
-      if (HASB && d->mlay == 1)
-        ew_tile<T, HASB, 16, HASB>(d, off, e, lA, lB);
-      else
-        ew_tile<T, HASB, 16>(d, off, e, lA, lB);
+      if constexpr (HASB) {
+        switch (d->mlay) {
+          case 1: ew_tile<T, true, 16, 1>(d, off, e, lA, lB); return;
+          case 2: ew_tile<T, true, 16, 2>(d, off, e, lA, lB); return;
+          case 3: ew_tile<T, true, 16, 3>(d, off, e, lA, lB); return;
+          default: break;
+        }
+      }
+      ew_tile<T, HASB, 16>(d, off, e, lA, lB);
       break;
   }
 }

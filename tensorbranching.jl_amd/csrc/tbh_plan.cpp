@@ -252,28 +252,41 @@ struct Builder {
     }
     std::sort(T.begin(), T.end(),
               [&](int32_t x, int32_t y) { return pos_in(LC, x) < pos_in(LC, y); });
-    // a full binary tile: tile bits 0..5 (the lanes) stay C's lowest tile
-    // positions (coalesced writes); bits 8, 9 become A-only and 10, 11
-    // B-only labels when the tile has them, so each thread's 16 outputs are
-    // a 4 x 4 outer product of 4 A and 4 B values per reduced configuration
+    // a full binary tile: tile bits 0..4 stay C's lowest tile positions (a
+    // lane's store covers whole 128-byte lines); bits 8..11 take MA A-only
+    // then 4 - MA B-only labels when the tile has them (MA = 2, else 3 or
+    // 1), so each thread's 16 outputs are an outer product of 2^MA A values
+    // and 2^(4-MA) B values per reduced configuration
     int mlay = 0;
     if (!op.direct && b >= 0 && (int)T.size() == kTileBits) {
-      std::vector<int32_t> ao, bo, rest;
-      for (int k = 6; k < kTileBits; k++) {
+      std::vector<int32_t> ao, bo, other;
+      for (int k = 5; k < kTileBits; k++) {
         const int32_t l = T[k];
-        if (inA(l) && !inB(l) && ao.size() < 2)
+        if (inA(l) && !inB(l))
           ao.push_back(l);
-        else if (inB(l) && !inA(l) && bo.size() < 2)
+        else if (inB(l) && !inA(l))
           bo.push_back(l);
         else
-          rest.push_back(l);
+          other.push_back(l);
       }
-      if (ao.size() == 2 && bo.size() == 2) {
-        T.resize(6);
-        T.insert(T.end(), rest.begin(), rest.end());
-        T.insert(T.end(), ao.begin(), ao.end());
-        T.insert(T.end(), bo.begin(), bo.end());
-        mlay = 1;
+      int ma = 0;
+      if (ao.size() >= 2 && bo.size() >= 2)
+        ma = 2;
+      else if (ao.size() >= 3 && bo.size() >= 1)
+        ma = 3;
+      else if (ao.size() >= 1 && bo.size() >= 3)
+        ma = 1;
+      if (ma) {
+        std::vector<int32_t> mid = other;  // bits 5..7
+        mid.insert(mid.end(), ao.begin() + ma, ao.end());
+        mid.insert(mid.end(), bo.begin() + (4 - ma), bo.end());
+        std::sort(mid.begin(), mid.end(),
+                  [&](int32_t x, int32_t y) { return pos_in(LC, x) < pos_in(LC, y); });
+        T.resize(5);
+        T.insert(T.end(), mid.begin(), mid.end());
+        T.insert(T.end(), ao.begin(), ao.begin() + ma);
+        T.insert(T.end(), bo.begin(), bo.begin() + (4 - ma));
+        mlay = ma;
       }
     }
     std::vector<int32_t> O;
@@ -444,11 +457,14 @@ int verify_plan(const SlicePlan& P, std::string* err) {
           return bad("EW tile bits");
       for (int k = d.nT; k < kTileBits; k++)
         if (d.tC[k] != -1 || d.tA[k] != -1 || d.tB[k] != -1) return bad("EW tile padding");
-      if (d.mlay != 0 &&
-          (d.mlay != 1 || op.direct || op.b < 0 || d.nT != kTileBits || d.tA[8] < 0 ||
-           d.tA[9] < 0 || d.tB[8] >= 0 || d.tB[9] >= 0 || d.tB[10] < 0 || d.tB[11] < 0 ||
-           d.tA[10] >= 0 || d.tA[11] >= 0))
-        return bad("EW outer-product tile layout");
+      if (d.mlay != 0) {
+        if (d.mlay < 1 || d.mlay > 3 || op.direct || op.b < 0 || d.nT != kTileBits)
+          return bad("EW outer-product tile layout");
+        for (int k = 8; k < kTileBits; k++) {
+          const bool a_only = d.tA[k] >= 0 && d.tB[k] < 0, b_only = d.tB[k] >= 0 && d.tA[k] < 0;
+          if (k < 8 + d.mlay ? !a_only : !b_only) return bad("EW outer-product tile layout");
+        }
+      }
       for (int i = 0; i < kMaxReduce; i++) {
         if (i >= d.nR) {
           if (d.rA[i] != -1 || d.rB[i] != -1) return bad("EW reduced padding");

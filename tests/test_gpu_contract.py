@@ -115,7 +115,8 @@ def test_slicing_invariant(tb, orc, backend):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.int32])
 def test_tile_variants_bit_exact(tb, orc, backend, dtype):
-    """A workload whose batched plan has 4 x 4 outer-product tiles, tiles
+    """A workload whose batched plan has outer-product tiles of every
+    layout (2 x 8, 4 x 4, 8 x 2 per thread), tiles
     with outer reduced labels (looped outside LDS), GEMM, direct and tiled
     steps: every slice's optimum equals the oracle's in both modes."""
     g = tb.random_regular_graph(160, 3, seed=5)
@@ -125,7 +126,7 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
     br = _slices(tb, g, 18, weights=w, seed=5, trials=30)
     steps = [s for b in br if b.code is not None for s in tb.plan_steps(b, dtype)]
     ew = [s for s in steps if s["kind"] == 0]
-    assert any(s["outer_product"] for s in ew)
+    assert {s["outer_product"] for s in ew} == {0, 1, 2, 3}  # every tile layout
     assert any(s["outer_reduced"] > 0 for s in ew)
     assert any(s["direct"] for s in ew) and any(not s["direct"] for s in ew)
     ref = orc.contract_slices(br, dtype)

@@ -19,3 +19,6 @@ from .workload import (  # noqa: F401
     ContractionTreeSlicer, dynamic_ob_mis, erdos_renyi, initialize_code, random_ksg,
     random_regular_graph, serving_requests, slice_graph)
 from .serve import Server  # noqa: F401
+from .io import (  # noqa: F401
+    load_all_finished, load_code, load_finished, load_weights, loadgraph, save_code,
+    save_finished, save_slices, save_weights, savegraph, threaded_saveslices)

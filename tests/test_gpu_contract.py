@@ -132,3 +132,17 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
     ref = orc.contract_slices(br, dtype)
     for mode in ("batched", "plain"):
         assert np.array_equal(backend.contract_slices(br, dtype, mode), ref), mode
+
+
+def test_slice_directory_on_gpu(tb, orc, backend, tmp_path):
+    """A slice directory in the reference's on-disk format (src/io.jl),
+    loaded with load_all_finished, contracts on the GPU to the oracle's
+    per-slice optima and the exact MIS (as test/slice.jl:22-35)."""
+    g = tb.random_regular_graph(100, 3, seed=0)
+    br = _slices(tb, g, 10)
+    tb.save_slices(str(tmp_path), br)
+    back = tb.load_all_finished(str(tmp_path))
+    ref = orc.contract_slices(back, np.float32)
+    for mode in ("batched", "plain"):
+        assert np.array_equal(backend.contract_slices(back, np.float32, mode), ref)
+    assert int(ref.max()) == orc.mis_exact(g)

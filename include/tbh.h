@@ -125,6 +125,10 @@ typedef struct {
   tbh_dtype dtype;
   const tbh_slice* slices;  /* copied at submit; caller may free afterwards */
   size_t n;
+  int32_t priority;         /* 0: normal; 1: latency lane (a caller blocked on
+                             * one slice, as slice_dfs_lp, src/slice.jl:39):
+                             * its slices go into the next round ahead of
+                             * every normal request's */
 } tbh_request;
 
 typedef enum {

@@ -26,7 +26,8 @@ class TbhSlice(C.Structure):
 
 
 class TbhRequest(C.Structure):
-    _fields_ = [("dtype", C.c_int), ("slices", C.POINTER(TbhSlice)), ("n", C.c_size_t)]
+    _fields_ = [("dtype", C.c_int), ("slices", C.POINTER(TbhSlice)), ("n", C.c_size_t),
+                ("priority", C.c_int32)]
 
 
 class TbhStats(C.Structure):

@@ -7,6 +7,7 @@
 //   planner    plans each submitted request's slices (BATCHED plans, on the
 //              context's plan threads) while the GPU runs earlier rounds;
 //   scheduler  forms rounds from the planned requests in arrival order --
+//              latency-lane requests (priority 1) ahead of the others, then
 //              the unscheduled slices of the oldest request first, then the
 //              next request's, up to max_resident_slices (and one dtype per
 //              round) -- runs each round as one BATCHED execution (its waves
@@ -35,6 +36,7 @@ double now_s() {
 struct Req {
   uint64_t ticket = 0;
   int dtype = TBH_F32;
+  int priority = 0;
   std::vector<SliceOwned> slices;
   std::vector<SlicePlan> plans;
   size_t next = 0;  // first slice not yet scheduled
@@ -118,9 +120,16 @@ struct ServeState {
         std::unique_lock<std::mutex> g(m);
         cv_sched.wait(g, [&] { return stop || !ready.empty(); });
         if (stop) return;
-        dtype = ready.front()->dtype;
+        // latency-lane requests first (in arrival order), then the rest;
+        // the round's dtype is that of its first request
+        std::vector<std::shared_ptr<Req>> order;
+        for (auto& r : ready)
+          if (r->priority) order.push_back(r);
+        for (auto& r : ready)
+          if (!r->priority) order.push_back(r);
+        dtype = order.front()->dtype;
         const size_t cap = max_resident ? max_resident : (size_t)-1;
-        for (auto& r : ready) {
+        for (auto& r : order) {
           if (r->dtype != dtype) continue;
           while (r->next < r->slices.size() && round.size() < cap) {
             round.push_back({r, r->next++});
@@ -259,6 +268,7 @@ int tbh_submit(tbh_ctx* ctx, const tbh_request* req, uint64_t* ticket) {
   }
   auto r = std::make_shared<tbh::Req>();
   r->dtype = req->dtype;
+  r->priority = req->priority ? 1 : 0;
   r->slices.resize(req->n);
   for (size_t i = 0; i < req->n; i++) {
     int rc = tbh::convert_slice(req->slices[i], req->dtype, true, &r->slices[i]);

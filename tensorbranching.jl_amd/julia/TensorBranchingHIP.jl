@@ -114,6 +114,55 @@ function solve_slice(b::SlicedBranch, element_type::Type, be::HIPBackend)
     return res[]
 end
 
+# ---- serving: continuous batching (tbh_serve_start / tbh_submit / tbh_wait)
+struct TbhRequest          # tbh_request
+    dtype::Cint
+    slices::Ptr{TbhSlice}
+    n::Csize_t
+    priority::Int32
+end
+
+start_serving(be::HIPBackend; max_resident_slices::Integer = 0) =
+    check(ccall((:tbh_serve_start, libtbh), Cint, (Ptr{Cvoid}, Csize_t), be.ctx,
+                max_resident_slices))
+stop_serving(be::HIPBackend) = check(ccall((:tbh_serve_stop, libtbh), Cint, (Ptr{Cvoid},), be.ctx))
+
+function check(rc::Cint)
+    rc == 0 || error(unsafe_string(ccall((:tbh_last_error, libtbh), Cstring, ())))
+    return nothing
+end
+
+# submits maximum(contract_slices(branches, T)) as one request; priority = 1
+# is the latency lane (a DFS caller blocked on one slice, src/slice.jl:39)
+function submit(be::HIPBackend, branches::Vector{<:SlicedBranch}, element_type::Type;
+                priority::Integer = 0)
+    bufs = [marshal(b, element_type) for b in branches]
+    ticket = Ref{UInt64}(0)
+    GC.@preserve bufs begin
+        slices = [TbhSlice(nv(b.p.g),
+                           buf[1] === nothing ? C_NULL : pointer(buf[1]),
+                           Int32(buf[5]), pointer(buf[2]), pointer(buf[3]), pointer(buf[4]),
+                           Float64(b.r)) for (b, buf) in zip(branches, bufs)]
+        GC.@preserve slices begin   # copied by tbh_submit
+            req = Ref(TbhRequest(dtype_code(element_type), pointer(slices), length(slices),
+                                 Int32(priority)))
+            check(ccall((:tbh_submit, libtbh), Cint, (Ptr{Cvoid}, Ptr{TbhRequest}, Ptr{UInt64}),
+                        be.ctx, req, ticket))
+        end
+    end
+    return ticket[]
+end
+
+function wait_result(be::HIPBackend, ticket::UInt64, element_type::Type)
+    res = Ref{element_type}()
+    status = Ref{Cint}(0)
+    check(ccall((:tbh_wait, libtbh), Cint, (Ptr{Cvoid}, UInt64, Ptr{Cvoid}, Ptr{Cint}),
+                be.ctx, ticket, res, status))
+    status[] == 2 || error("request $ticket failed: " *
+                           unsafe_string(ccall((:tbh_last_error, libtbh), Cstring, ())))
+    return res[]
+end
+
 # dynamic_ob_mis with the contraction step on the MI355X (src/dynamic_ob.jl:6-28)
 function dynamic_ob_mis(g, weights, be::HIPBackend; reducer = TensorBranching.TensorNetworkReducer(),
                         slicer = TensorBranching.ContractionTreeSlicer(),

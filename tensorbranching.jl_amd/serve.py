@@ -19,10 +19,11 @@ class Server:
         self._open = True
         self._dt = {}
 
-    def submit(self, branches, element_type=np.float32):
+    def submit(self, branches, element_type=np.float32, priority=0):
+        """priority=1: the latency lane (scheduled ahead of normal requests)."""
         code, npdt = _dtype_code(element_type)
         m = _Marshal(branches, npdt)
-        req = _lib.TbhRequest(code, m.ptr, m.n)
+        req = _lib.TbhRequest(code, m.ptr, m.n, int(priority))
         t = C.c_uint64()
         check(_lib.tbh().tbh_submit(self.be.handle, C.byref(req), C.byref(t)))
         self._dt[t.value] = npdt

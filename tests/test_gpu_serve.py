@@ -138,3 +138,29 @@ def test_submit_errors(tb, be):
     # after stop the serving entry points refuse
     with pytest.raises(tb.TbhError):
         _lib.check(_lib.tbh().tbh_cancel(be.handle, 1))
+
+
+def test_latency_lane(tb, orc, be):
+    """A priority request submitted behind a queue of large normal requests
+    is admitted into the next round: it finishes before most of them."""
+    big = []
+    for seed in range(6):
+        g = tb.random_regular_graph(100, 3, seed=20 + seed)
+        big.append(_sliced(tb, g, 8, seed=seed))
+    g1 = tb.random_regular_graph(40, 3, seed=30)
+    one = _sliced(tb, g1, 12, seed=30)[:1]
+    with tb.Server(be, max_resident_slices=8) as srv:
+        normal = [srv.submit(b) for b in big]
+        fast = srv.submit(one, priority=1)
+        done = {}
+        for t in normal + [fast]:
+            while True:
+                st, v, t0, t1 = srv.poll(t)
+                if st in ("done", "failed"):
+                    break
+                time.sleep(0.001)
+            assert st == "done"
+            done[t] = t1
+    later = sum(1 for t in normal if done[t] > done[fast])
+    assert later >= len(normal) // 2, (later, done)
+    assert sum(len(b) for b in big) > 6 * 8  # the queue spans many rounds

@@ -15,6 +15,16 @@ constexpr int kTileBits = 12;    // a tile has at most 2^12 outputs
 constexpr int kSubBits = 12;     // an LDS sub-tensor has at most 2^12 entries
 constexpr int kLdsElems = (1 << kSubBits) + ((1 << kSubBits) >> 5);  // padded
 
+// GEMM tile shape for an M x N output: tall-skinny outputs (N <= 64, M >=
+// 256) take a 256-row tile so that a narrow N still fills it.
+__host__ __device__ inline int gemm_shape(int64_t M, int64_t N) {
+  if (M >= 256 && N <= 32) return 2;
+  if (M >= 256 && N <= 64) return 1;
+  return 0;
+}
+__host__ __device__ inline int gemm_bm(int shape) { return shape ? 256 : 128; }
+__host__ __device__ inline int gemm_bn(int shape) { return shape == 2 ? 32 : shape == 1 ? 64 : 128; }
+
 // One EW step C = max over the reduced labels of A + B, tiled through LDS.
 // The output labels split into tile labels (nT bits, ordered by C position)
 // and outer labels (nO bits, one tile per configuration).  A tile first loads
@@ -58,6 +68,8 @@ struct GemmDesc {
   int64_t lda, ldb, ldc;
   int64_t sA, sB, sC;
   int32_t tiles_m, tiles_n; // tiles per batch entry
+  int32_t shape;            // tile 0: 128 x 128, 1: 256 x 64, 2: 256 x 32
+  int32_t pad_;
 };
 
 // Leaf tensor writes (vertex [0,w] or edge [0,0,0,-inf]).

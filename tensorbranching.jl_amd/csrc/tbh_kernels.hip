@@ -3,8 +3,9 @@
 // MFMA has no role.  Tropical Float32 uses -inf as zero; Tropical Int32 uses
 // the canonical sentinel -2^30 and rewrites stored values below -2^29 to it.
 //
-//   K1 tbh_gemm_kernel    batched / grouped max-plus GEMM: 128x128x16 LDS
-//                         tiles, 8x8 register blocking, add + max3 pairs.
+//   K1 tbh_gemm_kernel    batched / grouped max-plus GEMM: 128x128, 256x64 or
+//                         256x32 tiles, 16-deep LDS slabs, 8x8 / 8x4 register
+//                         blocking, add + max3 pairs.
 //   K2 tbh_ew_kernel      one wave-batched EW step: a binary tropical
 //                         contraction over bit-permuted 2^r tensors (labels of
 //                         size 2), reducing the contracted labels; also the
@@ -642,7 +643,7 @@ __global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW
 }
 
 // ---------------------------------------------------------------- K1 -----
-constexpr int BM = 128, BN = 128, BK = 16, BPAD = 4;
+constexpr int BK = 16, BPAD = 4;
 
 template <typename T>
 struct Vec4;
@@ -656,84 +657,124 @@ struct Vec4<int32_t> {
   using type = int32_t __attribute__((ext_vector_type(4)));
 };
 
-// Block of 256 threads computes a 128x128 tile of one batch entry; thread
-// (tx, ty) owns rows {tx*4..+3, 64+tx*4..+3} and the same pattern of columns.
-// VEC: A, B 16-byte aligned with lda, ldb, M, K multiples of 4.
-template <typename T, bool VEC>
-__device__ __forceinline__ void gemm_tile(const TBH_GLOBAL T* __restrict__ A,
-                                          const TBH_GLOBAL T* __restrict__ B,
-                                          TBH_GLOBAL T* __restrict__ C, int M, int N, int K, int64_t lda,
-                                          int64_t ldb, int64_t ldc, int m0, int n0,
-                                          T (*As)[BK][BM], T (*Bs)[BK][BN + BPAD]) {
+// Tile shapes (GemmDesc::shape): 128 x 128, and for tall-skinny steps
+// (N <= 64 or <= 32) 256 x 64 and 256 x 32, so that narrow outputs still
+// fill the tile.  Each of the 256 threads owns 8 rows x TN columns: rows
+// {tx*4 .. +3, BM/2 + tx*4 .. +3} (tx < BM/8), columns {ty*4 .. +3,
+// BN/2 + ty*4 .. +3} (TN = 8) or {ty*4 .. +3} (TN = 4).
+template <int SHAPE>
+struct GemmShape;
+template <>
+struct GemmShape<0> {
+  static constexpr int BM = 128, BN = 128, TN = 8;
+};
+template <>
+struct GemmShape<1> {
+  static constexpr int BM = 256, BN = 64, TN = 8;
+};
+template <>
+struct GemmShape<2> {
+  static constexpr int BM = 256, BN = 32, TN = 4;
+};
+
+// One BM x BN tile of one batch entry.  VEC: A, B 16-byte aligned with lda,
+// ldb, M, K multiples of 4.
+template <typename T, bool VEC, int SHAPE>
+__device__ __forceinline__ void gemm_tile(
+    const TBH_GLOBAL T* __restrict__ A, const TBH_GLOBAL T* __restrict__ B,
+    TBH_GLOBAL T* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb, int64_t ldc,
+    int m0, int n0, T (*As)[BK][GemmShape<SHAPE>::BM],
+    T (*Bs)[BK][GemmShape<SHAPE>::BN + BPAD]) {
+  constexpr int BM = GemmShape<SHAPE>::BM, BN = GemmShape<SHAPE>::BN, TN = GemmShape<SHAPE>::TN;
+  constexpr int TX = BM / 8;                    // threads along M
+  constexpr int NVA = BM * BK / 4 / 256;        // 4-vectors of A per thread
+  constexpr int NVBT = BN * BK / 4;             // 4-vectors of B per tile
+  constexpr int NVB = (NVBT + 255) / 256;       // ... per thread (some idle)
+  constexpr int NEA = BM * BK / 256;            // scalar loads per thread
+  constexpr int NEB = (BN * BK + 255) / 256;
   using V = typename Vec4<T>::type;
   const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
-  T acc[8][8];
+  const int tx = tid % TX, ty = tid / TX;
+  T acc[8][TN];
 #pragma unroll
   for (int i = 0; i < 8; i++)
 #pragma unroll
-    for (int j = 0; j < 8; j++) acc[i][j] = Trop<T>::zero();
+    for (int j = 0; j < TN; j++) acc[i][j] = Trop<T>::zero();
   const T pz = pad_zero<T>();
-  T ra[8], rb[8];
+  T ra[VEC ? 4 * NVA : NEA], rb[VEC ? 4 * NVB : NEB];
   auto load = [&](int k0) {
     if (VEC) {
-      // A: 2 x 4-vector per thread: m = (tid&31)*4, k = (tid>>5) + 8e
+      // A: vector v = tid + 256e covers m = (v % (BM/4))*4 .. +3, k = v / (BM/4)
 #pragma unroll
-      for (int e = 0; e < 2; e++) {
-        const int m = m0 + (tid & 31) * 4, k = k0 + (tid >> 5) + 8 * e;
-        V v;
+      for (int e = 0; e < NVA; e++) {
+        const int v = tid + 256 * e;
+        const int m = m0 + (v % (BM / 4)) * 4, k = k0 + v / (BM / 4);
+        V x;
         if (m < M && k < K)
-          v = *reinterpret_cast<const TBH_GLOBAL V*>(A + (int64_t)k * lda + m);
+          x = *reinterpret_cast<const TBH_GLOBAL V*>(A + (int64_t)k * lda + m);
         else
-          v = V{pz, pz, pz, pz};
-        ra[4 * e + 0] = v.x;
-        ra[4 * e + 1] = v.y;
-        ra[4 * e + 2] = v.z;
-        ra[4 * e + 3] = v.w;
+          x = V{pz, pz, pz, pz};
+        ra[4 * e + 0] = x.x;
+        ra[4 * e + 1] = x.y;
+        ra[4 * e + 2] = x.z;
+        ra[4 * e + 3] = x.w;
       }
-      // B: k = (tid&3)*4, n = (tid>>2) + 64e
+      // B: vector v covers k = (v & 3)*4 .. +3, n = v >> 2
 #pragma unroll
-      for (int e = 0; e < 2; e++) {
-        const int k = k0 + (tid & 3) * 4, n = n0 + (tid >> 2) + 64 * e;
-        V v;
-        if (n < N && k < K)
-          v = *reinterpret_cast<const TBH_GLOBAL V*>(B + (int64_t)n * ldb + k);
+      for (int e = 0; e < NVB; e++) {
+        const int v = tid + 256 * e;
+        const int k = k0 + (v & 3) * 4, n = n0 + (v >> 2);
+        V x;
+        if (v < NVBT && n < N && k < K)
+          x = *reinterpret_cast<const TBH_GLOBAL V*>(B + (int64_t)n * ldb + k);
         else
-          v = V{pz, pz, pz, pz};
-        rb[4 * e + 0] = v.x;
-        rb[4 * e + 1] = v.y;
-        rb[4 * e + 2] = v.z;
-        rb[4 * e + 3] = v.w;
+          x = V{pz, pz, pz, pz};
+        rb[4 * e + 0] = x.x;
+        rb[4 * e + 1] = x.y;
+        rb[4 * e + 2] = x.z;
+        rb[4 * e + 3] = x.w;
       }
     } else {
 #pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const int m = m0 + (tid & 127), k = k0 + (tid >> 7) + 2 * e;
+      for (int e = 0; e < NEA; e++) {
+        const int idx = tid + 256 * e;
+        const int m = m0 + idx % BM, k = k0 + idx / BM;
         ra[e] = (m < M && k < K) ? A[(int64_t)k * lda + m] : pz;
       }
 #pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const int k = k0 + (tid & 15), n = n0 + (tid >> 4) + 16 * e;
-        rb[e] = (n < N && k < K) ? B[(int64_t)n * ldb + k] : pz;
+      for (int e = 0; e < NEB; e++) {
+        const int idx = tid + 256 * e;
+        const int k = k0 + idx % BK, n = n0 + idx / BK;
+        rb[e] = (idx < BN * BK && n < N && k < K) ? B[(int64_t)n * ldb + k] : pz;
       }
     }
   };
   auto store = [&](int buf) {
     if (VEC) {
 #pragma unroll
-      for (int e = 0; e < 2; e++) {
-        *reinterpret_cast<V*>(&As[buf][(tid >> 5) + 8 * e][(tid & 31) * 4]) =
+      for (int e = 0; e < NVA; e++) {
+        const int v = tid + 256 * e;
+        *reinterpret_cast<V*>(&As[buf][v / (BM / 4)][(v % (BM / 4)) * 4]) =
             V{ra[4 * e], ra[4 * e + 1], ra[4 * e + 2], ra[4 * e + 3]};
       }
 #pragma unroll
-      for (int e = 0; e < 2; e++)
+      for (int e = 0; e < NVB; e++) {
+        const int v = tid + 256 * e;
+        if (v < NVBT)
 #pragma unroll
-        for (int q = 0; q < 4; q++) Bs[buf][(tid & 3) * 4 + q][(tid >> 2) + 64 * e] = rb[4 * e + q];
+          for (int q = 0; q < 4; q++) Bs[buf][(v & 3) * 4 + q][v >> 2] = rb[4 * e + q];
+      }
     } else {
 #pragma unroll
-      for (int e = 0; e < 8; e++) As[buf][(tid >> 7) + 2 * e][tid & 127] = ra[e];
+      for (int e = 0; e < NEA; e++) {
+        const int idx = tid + 256 * e;
+        As[buf][idx / BM][idx % BM] = ra[e];
+      }
 #pragma unroll
-      for (int e = 0; e < 8; e++) Bs[buf][tid & 15][(tid >> 4) + 16 * e] = rb[e];
+      for (int e = 0; e < NEB; e++) {
+        const int idx = tid + 256 * e;
+        if (idx < BN * BK) Bs[buf][idx % BK][idx / BK] = rb[e];
+      }
     }
   };
   load(0);
@@ -745,31 +786,29 @@ __device__ __forceinline__ void gemm_tile(const TBH_GLOBAL T* __restrict__ A,
     if (more) load(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 2) {
-      T a0[8], b0[8], a1[8], b1[8];
+      T a0[8], b0[TN], a1[8], b1[TN];
       V v;
-      v = *reinterpret_cast<const V*>(&As[buf][kk][tx * 4]);
-      a0[0] = v.x; a0[1] = v.y; a0[2] = v.z; a0[3] = v.w;
-      v = *reinterpret_cast<const V*>(&As[buf][kk][64 + tx * 4]);
-      a0[4] = v.x; a0[5] = v.y; a0[6] = v.z; a0[7] = v.w;
-      v = *reinterpret_cast<const V*>(&Bs[buf][kk][ty * 4]);
-      b0[0] = v.x; b0[1] = v.y; b0[2] = v.z; b0[3] = v.w;
-      v = *reinterpret_cast<const V*>(&Bs[buf][kk][64 + ty * 4]);
-      b0[4] = v.x; b0[5] = v.y; b0[6] = v.z; b0[7] = v.w;
-      v = *reinterpret_cast<const V*>(&As[buf][kk + 1][tx * 4]);
-      a1[0] = v.x; a1[1] = v.y; a1[2] = v.z; a1[3] = v.w;
-      v = *reinterpret_cast<const V*>(&As[buf][kk + 1][64 + tx * 4]);
-      a1[4] = v.x; a1[5] = v.y; a1[6] = v.z; a1[7] = v.w;
-      v = *reinterpret_cast<const V*>(&Bs[buf][kk + 1][ty * 4]);
-      b1[0] = v.x; b1[1] = v.y; b1[2] = v.z; b1[3] = v.w;
-      v = *reinterpret_cast<const V*>(&Bs[buf][kk + 1][64 + ty * 4]);
-      b1[4] = v.x; b1[5] = v.y; b1[6] = v.z; b1[7] = v.w;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        v = *reinterpret_cast<const V*>(&As[buf][kk][h * (BM / 2) + tx * 4]);
+        a0[4 * h] = v.x; a0[4 * h + 1] = v.y; a0[4 * h + 2] = v.z; a0[4 * h + 3] = v.w;
+        v = *reinterpret_cast<const V*>(&As[buf][kk + 1][h * (BM / 2) + tx * 4]);
+        a1[4 * h] = v.x; a1[4 * h + 1] = v.y; a1[4 * h + 2] = v.z; a1[4 * h + 3] = v.w;
+      }
+#pragma unroll
+      for (int h = 0; h < TN / 4; h++) {
+        v = *reinterpret_cast<const V*>(&Bs[buf][kk][h * (BN / 2) + ty * 4]);
+        b0[4 * h] = v.x; b0[4 * h + 1] = v.y; b0[4 * h + 2] = v.z; b0[4 * h + 3] = v.w;
+        v = *reinterpret_cast<const V*>(&Bs[buf][kk + 1][h * (BN / 2) + ty * 4]);
+        b1[4 * h] = v.x; b1[4 * h + 1] = v.y; b1[4 * h + 2] = v.z; b1[4 * h + 3] = v.w;
+      }
 #pragma unroll
       for (int i = 0; i < 8; i++)
 #pragma unroll
-        for (int j = 0; j < 8; j++)
+        for (int j = 0; j < TN; j++)
           acc[i][j] = max3(acc[i][j], Trop<T>::mul(a0[i], b0[j]), Trop<T>::mul(a1[i], b1[j]));
       // one k pair at a time: keeps the LDS values of later pairs from being
-      // hoisted into registers (the 8x8 maxima already take 64)
+      // hoisted into registers
       __builtin_amdgcn_sched_barrier(0);
     }
     if (more) store(buf ^ 1);
@@ -777,13 +816,13 @@ __device__ __forceinline__ void gemm_tile(const TBH_GLOBAL T* __restrict__ A,
     buf ^= 1;
   }
 #pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const int n = n0 + (j < 4 ? ty * 4 + j : 64 + ty * 4 + (j - 4));
+  for (int j = 0; j < TN; j++) {
+    const int n = n0 + (j < 4 ? ty * 4 + j : BN / 2 + ty * 4 + (j - 4));
     if (n >= N) continue;
     if (VEC) {  // M and ldc are multiples of 4: rows come in aligned quads
 #pragma unroll
       for (int h = 0; h < 2; h++) {
-        const int m = m0 + h * 64 + tx * 4;
+        const int m = m0 + h * (BM / 2) + tx * 4;
         if (m < M)
           *reinterpret_cast<TBH_GLOBAL V*>(C + (int64_t)n * ldc + m) =
               V{Trop<T>::canon(acc[4 * h][j]), Trop<T>::canon(acc[4 * h + 1][j]),
@@ -792,7 +831,7 @@ __device__ __forceinline__ void gemm_tile(const TBH_GLOBAL T* __restrict__ A,
     } else {
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        const int m = m0 + (i < 4 ? tx * 4 + i : 64 + tx * 4 + (i - 4));
+        const int m = m0 + (i < 4 ? tx * 4 + i : BM / 2 + tx * 4 + (i - 4));
         if (m < M) C[(int64_t)n * ldc + m] = Trop<T>::canon(acc[i][j]);
       }
     }
@@ -800,10 +839,12 @@ __device__ __forceinline__ void gemm_tile(const TBH_GLOBAL T* __restrict__ A,
 }
 
 // Grouped: prefix[i] is the first tile of GEMM step i; a block walks tiles.
-template <typename T, bool VEC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBH_GEMM_WAVES))) void tbh_gemm_kernel(const GemmDesc* __restrict__ ops,
-                                                       const int64_t* __restrict__ prefix,
-                                                       int nops, int64_t total) {
+// Every step of one launch has the tile shape SHAPE.
+template <typename T, bool VEC, int SHAPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBH_GEMM_WAVES))) void
+tbh_gemm_kernel(const GemmDesc* __restrict__ ops, const int64_t* __restrict__ prefix, int nops,
+                int64_t total) {
+  constexpr int BM = GemmShape<SHAPE>::BM, BN = GemmShape<SHAPE>::BN;
   __shared__ T As[2][BK][BM];
   __shared__ T Bs[2][BK][BN + BPAD];
   for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
@@ -824,7 +865,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBH_GEMM_WA
     const TBH_GLOBAL T* A = reinterpret_cast<const TBH_GLOBAL T*>(d->a) + t * d->sA;
     const TBH_GLOBAL T* B = reinterpret_cast<const TBH_GLOBAL T*>(d->b) + t * d->sB;
     TBH_GLOBAL T* C = reinterpret_cast<TBH_GLOBAL T*>(d->c) + t * d->sC;
-    gemm_tile<T, VEC>(A, B, C, d->M, d->N, d->K, d->lda, d->ldb, d->ldc, tm * BM, tn * BN, As, Bs);
+    gemm_tile<T, VEC, SHAPE>(A, B, C, d->M, d->N, d->K, d->lda, d->ldb, d->ldc, tm * BM, tn * BN,
+                             As, Bs);
     __syncthreads();
   }
 }
@@ -922,20 +964,32 @@ hipError_t launch_ew_direct(int dtype, const EwDesc* ops, const int64_t* prefix,
   return hipGetLastError();
 }
 
-hipError_t launch_gemm(int dtype, bool vec, const GemmDesc* ops, const int64_t* prefix, int nops,
-                       int64_t total, int grid, hipStream_t s) {
+hipError_t launch_gemm(int dtype, bool vec, int shape, const GemmDesc* ops, const int64_t* prefix,
+                       int nops, int64_t total, int grid, hipStream_t s) {
   if (total <= 0) return hipSuccess;
+#define TBH_GEMM_LAUNCH(T, V, S) tbh_gemm_kernel<T, V, S><<<grid, 256, 0, s>>>(ops, prefix, nops, total)
+#define TBH_GEMM_SHAPES(T, V)      \
+  if (shape == 1)                  \
+    TBH_GEMM_LAUNCH(T, V, 1);      \
+  else if (shape == 2)             \
+    TBH_GEMM_LAUNCH(T, V, 2);      \
+  else                             \
+    TBH_GEMM_LAUNCH(T, V, 0);
   if (dtype == 0) {
-    if (vec)
-      tbh_gemm_kernel<float, true><<<grid, 256, 0, s>>>(ops, prefix, nops, total);
-    else
-      tbh_gemm_kernel<float, false><<<grid, 256, 0, s>>>(ops, prefix, nops, total);
+    if (vec) {
+      TBH_GEMM_SHAPES(float, true)
+    } else {
+      TBH_GEMM_SHAPES(float, false)
+    }
   } else {
-    if (vec)
-      tbh_gemm_kernel<int32_t, true><<<grid, 256, 0, s>>>(ops, prefix, nops, total);
-    else
-      tbh_gemm_kernel<int32_t, false><<<grid, 256, 0, s>>>(ops, prefix, nops, total);
+    if (vec) {
+      TBH_GEMM_SHAPES(int32_t, true)
+    } else {
+      TBH_GEMM_SHAPES(int32_t, false)
+    }
   }
+#undef TBH_GEMM_SHAPES
+#undef TBH_GEMM_LAUNCH
   return hipGetLastError();
 }
 

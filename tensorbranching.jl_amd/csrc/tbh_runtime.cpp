@@ -347,8 +347,10 @@ GemmDesc make_gemm(const POp& op, const std::vector<PTensor>& T, uint64_t base) 
   d.sA = (int64_t)op.M * op.K;
   d.sB = (int64_t)op.K * op.N;
   d.sC = (int64_t)op.M * op.N;
-  d.tiles_m = (op.M + 127) / 128;
-  d.tiles_n = (op.N + 127) / 128;
+  d.shape = gemm_shape(op.M, op.N);
+  d.pad_ = 0;
+  d.tiles_m = (op.M + gemm_bm(d.shape) - 1) / gemm_bm(d.shape);
+  d.tiles_n = (op.N + gemm_bn(d.shape) - 1) / gemm_bn(d.shape);
   return d;
 }
 
@@ -393,12 +395,18 @@ int ew_chunk(tbh_ctx*, int64_t) {
 // ------------------------------------------------------ BATCHED ----------
 namespace {
 
-int op_kind(const POp& op) { return op.kind == kGemm ? 2 : (op.direct ? 0 : 1); }
-int64_t op_tiles(const POp& op) {
-  return op.kind == kGemm ? (int64_t)((op.M + 127) / 128) * ((op.N + 127) / 128) * op.batch
-                          : op.tiles;
+// launch kinds: 0 direct EW, 1 tiled EW, 2 + s GEMM with tile shape s
+constexpr int kKinds = 5;
+int op_kind(const POp& op) {
+  return op.kind == kGemm ? 2 + gemm_shape(op.M, op.N) : (op.direct ? 0 : 1);
 }
-size_t desc_size(int kind) { return kind == 2 ? sizeof(GemmDesc) : sizeof(EwDesc); }
+int64_t op_tiles(const POp& op) {
+  if (op.kind != kGemm) return op.tiles;
+  const int sh = gemm_shape(op.M, op.N), bm = gemm_bm(sh), bn = gemm_bn(sh);
+  return (int64_t)((op.M + bm - 1) / bm) * ((op.N + bn - 1) / bn) * op.batch;
+}
+size_t desc_size(int kind) { return kind >= 2 ? sizeof(GemmDesc) : sizeof(EwDesc); }
+int stat_kind(int kind) { return kind < 2 ? kind : 2; }  // tbh_stats: direct, tiled, GEMM
 
 // Runs f(i) for i in [0, n) on up to `threads` host threads.
 template <typename F>
@@ -520,7 +528,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
         H = std::max(H, plans[i]->nlevels);
       }
     }
-    const int W = 3 * (H + 1);  // (kind, level) slots
+    const int W = kKinds * (H + 1);  // (kind, level) slots
     auto slot = [H](int k, int L) { return k * (H + 1) + L; };
     // pass 1
     std::vector<int32_t> cnt(W, 0), start(nb * W);
@@ -612,7 +620,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
       for (const POp& op : P.ops) {
         const int k = op_kind(op), w = slot(k, op.level);
         const int32_t idx = st[w] + loc[w]++;
-        if (k == 2)
+        if (k >= 2)
           reinterpret_cast<GemmDesc*>(hb + o_desc[w])[idx] = make_gemm(op, P.tensors, b);
         else
           reinterpret_cast<EwDesc*>(hb + o_desc[w])[idx] = make_ew(op, P.tensors, b);
@@ -648,18 +656,18 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
         return launch_ew(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
                          ew_grid(ctx, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
                          ctx->stream);
-      return launch_gemm(dtype, true, reinterpret_cast<const GemmDesc*>(descs), prefix, nops, total,
-                         gemm_grid(ctx, total), ctx->stream);
+      return launch_gemm(dtype, true, k - 2, reinterpret_cast<const GemmDesc*>(descs), prefix,
+                         nops, total, gemm_grid(ctx, total), ctx->stream);
     };
     for (int L = 1; L <= H && e == hipSuccess; L++) {
-      for (int k = 0; k < 3 && e == hipSuccess; k++) {
+      for (int k = 0; k < kKinds && e == hipSuccess; k++) {
         const int w = slot(k, L);
         if (!cnt[w]) continue;
         const int64_t total = pfx[w].back();
         if (per_step) {
           for (int32_t q = 0; q < cnt[w] && e == hipSuccess; q++) {
             char tag[160];
-            if (k == 2) {
+            if (k >= 2) {
               const GemmDesc& d = reinterpret_cast<const GemmDesc*>(hb + o_desc[w])[q];
               std::snprintf(tag, sizeof tag, "M=%d N=%d K=%d batch=%d", d.M, d.N, d.K, d.batch);
             } else {
@@ -672,8 +680,8 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
             ev = ctx->prof.begin(ctx->stream);
             e = launch(k, dbase + o_desc[w] + q * desc_size(k),
                        reinterpret_cast<const int64_t*>(dbase + o_one[w]) + 2 * q, 1, tiles);
-            ctx->prof.end(ctx->stream, ev, k, L, 1, step_work[w][q].first, step_work[w][q].second,
-                          tag);
+            ctx->prof.end(ctx->stream, ev, stat_kind(k), L, 1, step_work[w][q].first,
+                          step_work[w][q].second, tag);
             ctx->stats.launches++;
           }
           continue;
@@ -681,7 +689,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
         if (prof) ev = ctx->prof.begin(ctx->stream);
         e = launch(k, dbase + o_desc[w], reinterpret_cast<const int64_t*>(dbase + o_pfx[w]),
                    cnt[w], total);
-        if (prof) ctx->prof.end(ctx->stream, ev, k, L, cnt[w], lvl_bytes[w], lvl_macs[w]);
+        if (prof) ctx->prof.end(ctx->stream, ev, stat_kind(k), L, cnt[w], lvl_bytes[w], lvl_macs[w]);
         ctx->stats.launches++;
       }
       ctx->stats.waves++;
@@ -809,7 +817,7 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
       ctx->stats.ew_ops++;
       ctx->stats.ew_ops_algo_bytes += op.bytes;
     } else {
-      err = launch_gemm(dtype, true, (const GemmDesc*)(dbase + o_op[k]),
+      err = launch_gemm(dtype, true, gemm_shape(op.M, op.N), (const GemmDesc*)(dbase + o_op[k]),
                         (const int64_t*)(dbase + o_pre[k]), 1, tiles[k], gemm_grid(ctx, tiles[k]),
                         ctx->stream);
       ctx->stats.gemm_ops++;
@@ -1052,8 +1060,10 @@ int tbh_maxplus_gemm(tbh_ctx* ctx, tbh_dtype dtype, int64_t M, int64_t N, int64_
   d.sA = strideA;
   d.sB = strideB;
   d.sC = strideC;
-  d.tiles_m = (int32_t)((M + 127) / 128);
-  d.tiles_n = (int32_t)((N + 127) / 128);
+  d.shape = gemm_shape(M, N);
+  d.pad_ = 0;
+  d.tiles_m = (int32_t)((M + gemm_bm(d.shape) - 1) / gemm_bm(d.shape));
+  d.tiles_n = (int32_t)((N + gemm_bn(d.shape) - 1) / gemm_bn(d.shape));
   const int64_t total = gemm_tiles(d);
   const bool vec = M % 4 == 0 && K % 4 == 0 && lda % 4 == 0 && ldb % 4 == 0 && ldc % 4 == 0 &&
                    strideA % 4 == 0 && strideB % 4 == 0 && strideC % 4 == 0 &&
@@ -1078,8 +1088,8 @@ int tbh_maxplus_gemm(tbh_ctx* ctx, tbh_dtype dtype, int64_t M, int64_t N, int64_
                                 hipMemcpyHostToDevice, ctx->stream);
   hipEvent_t ev = ctx->profiling ? ctx->prof.begin(ctx->stream) : nullptr;
   if (e == hipSuccess)
-    e = launch_gemm(dtype, vec, (const GemmDesc*)dd, (const int64_t*)(dd + sizeof d), 1, total,
-                    gemm_grid(ctx, total), ctx->stream);
+    e = launch_gemm(dtype, vec, d.shape, (const GemmDesc*)dd, (const int64_t*)(dd + sizeof d), 1,
+                    total, gemm_grid(ctx, total), ctx->stream);
   if (ev)
     ctx->prof.end(ctx->stream, ev, 2, 0, 1, 4.0 * (double)batch * (M * K + K * N + M * N),
                   (double)M * N * K * batch);

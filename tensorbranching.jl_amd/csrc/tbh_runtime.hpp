@@ -27,8 +27,8 @@ hipError_t launch_ew(int dtype, const EwDesc* ops, const int64_t* prefix, int no
                      int grid, int chunk, hipStream_t s);
 hipError_t launch_ew_direct(int dtype, const EwDesc* ops, const int64_t* prefix, int nops,
                             int64_t total, int grid, int chunk, hipStream_t s);
-hipError_t launch_gemm(int dtype, bool vec, const GemmDesc* ops, const int64_t* prefix, int nops,
-                       int64_t total, int grid, hipStream_t s);
+hipError_t launch_gemm(int dtype, bool vec, int shape, const GemmDesc* ops, const int64_t* prefix,
+                       int nops, int64_t total, int grid, hipStream_t s);
 hipError_t launch_leaves(int dtype, const LeafDesc* leaves, int64_t n, hipStream_t s);
 hipError_t launch_epilogue(int dtype, const EpiDesc* e, int64_t n, void* out, hipStream_t s);
 hipError_t launch_valu_probe(int dtype, void* out, int blocks, int iters, hipStream_t s);

@@ -9,7 +9,10 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [(1, 1, 1), (3, 5, 7), (16, 16, 16), (64, 64, 64), (100, 37, 129), (128, 128, 16),
           (256, 64, 512), (16, 4096, 16), (4096, 16, 64), (512, 512, 256), (130, 257, 3),
-          (2048, 256, 256)]
+          (2048, 256, 256),
+          # tall-skinny tiles (256 x 64, 256 x 32): ragged M, N, K, and K not a
+          # multiple of 4 (scalar loads)
+          (300, 33, 20), (257, 64, 128), (1000, 32, 17), (4100, 7, 9), (512, 65, 64)]
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.int32])
@@ -23,6 +26,14 @@ def test_gemm_matches_oracle(backend, orc, dtype, shape):
     ref = orc.maxplus_gemm(A, B)
     assert got.dtype == ref.dtype
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.int32])
+def test_gemm_batched_skinny(backend, orc, dtype):
+    rng = np.random.default_rng(6)
+    A = rand_tropical(rng, (768, 40, 3), dtype)
+    B = rand_tropical(rng, (40, 48, 3), dtype)
+    assert np.array_equal(backend.maxplus_gemm(A, B), orc.maxplus_gemm(A, B))
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.int32])

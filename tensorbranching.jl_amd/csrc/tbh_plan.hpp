@@ -79,7 +79,7 @@ struct PlanOptions {
   // chain_big absorbing side tensors of rank <= chain_side, combined side
   // tensor <= chain_side, at most chain_reduce labels reduced per step
   bool reassociate = true;
-  int chain_big = 16, chain_side = 15, chain_reduce = 5;
+  int chain_big = 17, chain_side = 16, chain_reduce = 7;
 };
 
 // Returns 0 or sets *err and returns -1.

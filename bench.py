@@ -6,6 +6,10 @@ their inputs on the host side of the C ABI (the backend plans and uploads
 inside the step).  Workload: random 3-regular graph, n = 300, seed 2, sliced
 by the host slicer to sc_target = 25, Tropical Float32, unit weights.
 
+Before the timed steps the autotuner (tbh_autotune) times up to 12 candidate
+planner / launch settings on the workload and installs the fastest (untimed,
+like a warmup); the choice is printed.
+
 Printed on one JSON line: solves/sec (value), the plain per-slice GPU loop on
 the same slices, the dominant kernel's algorithmic throughput against its
 roofline (HIP events on the backend's stream), a standalone max-plus GEMM
@@ -273,6 +277,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune-trials", type=int, default=12,
+                    help="contractions the autotuner may time before the timed steps (0: off)")
     ap.add_argument("--serve-copies", type=int, default=3)
     ap.add_argument("--serve-mix", type=int, default=24)
     ap.add_argument("--no-serve", action="store_true")
@@ -295,6 +301,14 @@ def main():
     for _ in range(args.warmup):
         r = be.contract_slices(br, np.float32, "batched")
         ref = r if ref is None else ref
+    # autotuner (untimed): planner / launch parameters for this workload
+    tune = None
+    if args.autotune_trials > 0 and not args.batched_only:
+        t_tune = time.perf_counter()
+        tuning, tuned_ms = be.autotune(br, np.float32, max_trials=args.autotune_trials)
+        tune = {"trials": args.autotune_trials, "chosen": tuning, "ms": round(tuned_ms, 1),
+                "seconds": round(time.perf_counter() - t_tune, 1)}
+        log(f"autotune: {tune}")
     # timed steps (the call is synchronous: results are on the host at return)
     be.reset_stats()
     lat = []
@@ -416,6 +430,7 @@ def main():
         "plain_gpu_loop": None if t_plain is None else {
             "solves_per_s": round(1.0 / t_plain, 4), "ms_per_solve": round(1e3 * t_plain, 1),
             "batched_speedup": round(solves_per_s * t_plain, 2)},
+        "autotune": tune,
         "serving": serving,
         "tropical_GOPs": round(st_timed["total_algo_ops"] / t_total / 1e9, 1),
         "roofline": roofline,

@@ -177,6 +177,31 @@ typedef struct {
 int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode,
                    tbh_step_info* out, int cap);
 
+/* ---- tuning ------------------------------------------------------------ */
+/* Planner and launch parameters of the BATCHED executor (no reference
+ * counterpart).  chain_*: spine-chain re-association (spine rank >=
+ * chain_big absorbs side tensors of rank <= chain_side, <= chain_reduce
+ * labels reduced per step); split_*: GEMM split-K (up to 2^split_tiles_bits
+ * tiles per node, K parts >= 2^split_min_k_bits); ew_chunk / direct_chunk:
+ * tiles per claim of the tiled / direct EW kernels; first_batch: fraction
+ * of the arena region the first batch may use.  Results never depend on
+ * them. */
+typedef struct {
+  int32_t chain_big, chain_side, chain_reduce;
+  int32_t split_tiles_bits, split_min_k_bits;
+  int32_t ew_chunk, direct_chunk;
+  int32_t pad_;
+  double first_batch;
+} tbh_tuning;
+int tbh_get_tuning(tbh_ctx* ctx, tbh_tuning* out);
+int tbh_set_tuning(tbh_ctx* ctx, const tbh_tuning* t);
+/* Autotuner: times BATCHED contractions of the given slices under candidate
+ * tunings (coordinate descent from the current one, at most max_trials
+ * contractions, each candidate's results checked equal to the first's),
+ * installs the fastest on the context and returns it with its wall time. */
+int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t n,
+                 int max_trials, tbh_tuning* best, double* best_ms);
+
 /* Measured VALU ceiling of the max-plus instruction mix (no reference
  * counterpart: the denominator of the GEMM's roofline).  Runs a
  * register-only kernel with the GEMM inner loop's mix -- 2 adds and 1 max3

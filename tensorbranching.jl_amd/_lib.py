@@ -40,6 +40,13 @@ class TbhStats(C.Structure):
                 ("kind_launches", C.c_uint64 * 3), ("host_ms", C.c_double * 4)]
 
 
+class TbhTuning(C.Structure):
+    _fields_ = [("chain_big", C.c_int32), ("chain_side", C.c_int32), ("chain_reduce", C.c_int32),
+                ("split_tiles_bits", C.c_int32), ("split_min_k_bits", C.c_int32),
+                ("ew_chunk", C.c_int32), ("direct_chunk", C.c_int32), ("pad_", C.c_int32),
+                ("first_batch", C.c_double)]
+
+
 class TbhStepInfo(C.Structure):
     _fields_ = [("kind", C.c_int32), ("level", C.c_int32), ("rank_a", C.c_int32),
                 ("rank_b", C.c_int32), ("rank_c", C.c_int32), ("n_reduced", C.c_int32),
@@ -77,6 +84,10 @@ TBH_SIGNATURES = {
     "tbh_all_memory": (C.c_int, [C.POINTER(TbhSlice), C.POINTER(C.c_double)]),
     "tbh_slice_footprint": (C.c_int, [C.POINTER(TbhSlice), C.c_int, C.POINTER(C.c_size_t)]),
     "tbh_valu_peak": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "tbh_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(TbhTuning)]),
+    "tbh_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(TbhTuning)]),
+    "tbh_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(TbhSlice), C.c_size_t, C.c_int,
+                               C.POINTER(TbhTuning), C.POINTER(C.c_double)]),
     "tbh_serve_start": (C.c_int, [C.c_void_p, C.c_size_t]),
     "tbh_serve_stop": (C.c_int, [C.c_void_p]),
     "tbh_submit": (C.c_int, [C.c_void_p, C.POINTER(TbhRequest), C.POINTER(C.c_uint64)]),

@@ -275,6 +275,32 @@ class HIPBackend:
                                           _mode(mode)))
         return out[0]
 
+    def tuning(self):
+        """The context's planner / launch parameters (tbh_tuning) as a dict."""
+        t = _lib.TbhTuning()
+        check(_lib.tbh().tbh_get_tuning(self.handle, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_ if k != "pad_"}
+
+    def set_tuning(self, **kw):
+        t = _lib.TbhTuning()
+        check(_lib.tbh().tbh_get_tuning(self.handle, C.byref(t)))
+        for k, v in kw.items():
+            if k not in dict(t._fields_) or k == "pad_":
+                raise KeyError(k)
+            setattr(t, k, v)
+        check(_lib.tbh().tbh_set_tuning(self.handle, C.byref(t)))
+
+    def autotune(self, branches, element_type=np.float32, max_trials=16):
+        """Times BATCHED contractions of `branches` under candidate tunings,
+        installs the fastest; returns (tuning dict, ms of one contraction)."""
+        code, npdt = _dtype_code(element_type)
+        m = _Marshal(branches, npdt)
+        t = _lib.TbhTuning()
+        ms = C.c_double()
+        check(_lib.tbh().tbh_autotune(self.handle, code, m.ptr, m.n, int(max_trials), C.byref(t),
+                                      C.byref(ms)))
+        return {k: getattr(t, k) for k, _ in t._fields_ if k != "pad_"}, ms.value
+
     def valu_peak(self, element_type=np.float32):
         """Measured tropical ops/s of the GEMM's VALU instruction mix with
         register operands (the roofline ceiling of compute-bound steps)."""

@@ -665,12 +665,7 @@ int slice_peak_memory(const SliceIn& s, double* peak, double* all, std::string* 
 }
 
 int plan_slice(const SliceIn& s_in, const PlanOptions& opt_in, SlicePlan* out, std::string* err) {
-  PlanOptions opt = opt_in;
-  // tuning overrides: TBH_CHAIN="big,side,reduce", TBH_SPLITK="tile_bits,min_k_bits,max_rank"
-  if (const char* e = std::getenv("TBH_CHAIN"))
-    std::sscanf(e, "%d,%d,%d", &opt.chain_big, &opt.chain_side, &opt.chain_reduce);
-  if (const char* e = std::getenv("TBH_SPLITK"))
-    std::sscanf(e, "%d,%d,%d", &opt.split_tiles_bits, &opt.split_min_k_bits, &opt.split_max_rank);
+  const PlanOptions& opt = opt_in;
   SlicePlan& P = *out;
   P = SlicePlan();
   if (validate(s_in, err)) return -1;

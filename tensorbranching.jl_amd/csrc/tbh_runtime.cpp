@@ -220,11 +220,47 @@ int convert_slice(const tbh_slice& s, tbh_dtype dtype, bool copy, SliceOwned* ou
   return 0;
 }
 
+tbh_tuning default_tuning() {
+  static const tbh_tuning t = [] {
+    tbh_tuning d{};
+    const PlanOptions o;
+    d.chain_big = o.chain_big;
+    d.chain_side = o.chain_side;
+    d.chain_reduce = o.chain_reduce;
+    d.split_tiles_bits = o.split_tiles_bits;
+    d.split_min_k_bits = o.split_min_k_bits;
+    d.ew_chunk = 2;
+    d.direct_chunk = 4;
+    d.first_batch = 0.15;
+    if (const char* e = std::getenv("TBH_CHAIN"))
+      std::sscanf(e, "%d,%d,%d", &d.chain_big, &d.chain_side, &d.chain_reduce);
+    if (const char* e = std::getenv("TBH_SPLITK"))
+      std::sscanf(e, "%d,%d", &d.split_tiles_bits, &d.split_min_k_bits);
+    if (const char* e = std::getenv("TBH_EW_CHUNK")) d.ew_chunk = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("TBH_DIRECT_CHUNK")) d.direct_chunk = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("TBH_FIRST_BATCH"))
+      d.first_batch = std::min(1.0, std::max(0.01, std::atof(e)));
+    return d;
+  }();
+  return t;
+}
+
+PlanOptions plan_options(const tbh_ctx* ctx, bool batched) {
+  PlanOptions o;
+  o.batched = batched;
+  const tbh_tuning& t = ctx->tune;
+  o.chain_big = t.chain_big;
+  o.chain_side = t.chain_side;
+  o.chain_reduce = t.chain_reduce;
+  o.split_tiles_bits = t.split_tiles_bits;
+  o.split_min_k_bits = t.split_min_k_bits;
+  return o;
+}
+
 int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool batched,
              std::vector<SlicePlan>* plans) {
   plans->assign(slices.size(), SlicePlan());
-  PlanOptions opt;
-  opt.batched = batched;
+  const PlanOptions opt = plan_options(ctx, batched);
   std::atomic<size_t> next{0};
   std::atomic<int> bad{-1};
   std::string first_err;
@@ -258,10 +294,9 @@ int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool ba
   return 0;
 }
 
-PlanStream::PlanStream(const std::vector<const SliceOwned*>& slices, bool batched, int threads)
+PlanStream::PlanStream(const std::vector<const SliceOwned*>& slices, const PlanOptions& opt,
+                       int threads)
     : slices_(slices), plans_(slices.size()), done_(slices.size(), 0) {
-  PlanOptions opt;
-  opt.batched = batched;
   auto worker = [this, opt]() {
     for (;;) {
       const size_t i = next_.fetch_add(1);
@@ -375,20 +410,8 @@ int direct_grid(tbh_ctx* ctx, int64_t total, int chunk) {
 // Waves (direct) and workgroups (tiled) take small chunks of the tile space
 // round-robin, which spreads expensive steps evenly; long contiguous ranges
 // measured slower (costly steps cluster in a few ranges).
-int direct_chunk(tbh_ctx*, int64_t) {
-  static const int c = [] {
-    const char* e = std::getenv("TBH_DIRECT_CHUNK");  // tuning sweeps
-    return e ? std::max(1, std::atoi(e)) : 4;
-  }();
-  return c;
-}
-int ew_chunk(tbh_ctx*, int64_t) {
-  static const int c = [] {
-    const char* e = std::getenv("TBH_EW_CHUNK");  // tuning sweeps
-    return e ? std::max(1, std::atoi(e)) : 2;
-  }();
-  return c;
-}
+int direct_chunk(tbh_ctx* ctx, int64_t) { return std::max(1, ctx->tune.direct_chunk); }
+int ew_chunk(tbh_ctx* ctx, int64_t) { return std::max(1, ctx->tune.ew_chunk); }
 
 }  // namespace
 
@@ -487,10 +510,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
   };
   // the first batch is kept to a fraction of the region so that the GPU
   // starts while the later slices are still being planned
-  static const double first_frac = [] {
-    const char* e = std::getenv("TBH_FIRST_BATCH");  // tuning sweeps
-    return e ? std::min(1.0, std::max(0.01, std::atof(e))) : 0.15;
-  }();
+  const double first_frac = std::min(1.0, std::max(0.01, ctx->tune.first_batch));
   size_t i0 = 0;
   for (int bidx = 0; i0 < n && status == 0; bidx++) {
     auto t_plan = std::chrono::steady_clock::now();
@@ -895,6 +915,7 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
   ctx->pool.init(pool_bytes);
   ctx->stats.pool_bytes = ctx->pool.capacity();
   ctx->plan_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  ctx->tune = default_tuning();
   *out = ctx.release();
   return 0;
 }
@@ -938,7 +959,7 @@ static int contract_impl(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
   uint32_t* out = (uint32_t*)results;
   if (mode == TBH_BATCHED) {
     // slices are planned on host threads while earlier batches run
-    PlanStream ps(ptrs, true, ctx->plan_threads);
+    PlanStream ps(ptrs, plan_options(ctx, true), ctx->plan_threads);
     ctx->stats.host_ms[0] += ms_since(t_plan);
     return execute_batched(ctx, dtype, ptrs,
                            [&](size_t i, std::string* err) { return ps.get(i, err); }, add_r, out);
@@ -1223,6 +1244,158 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode, tbh_s
     s.bytes = op.bytes;
   }
   return n;
+}
+
+static bool serving_active(tbh_ctx* ctx) {
+  std::lock_guard<std::mutex> g(ctx->serve_mu);
+  return (bool)ctx->serve;
+}
+
+static int check_tuning(const tbh_tuning& t) {
+  if (t.chain_big < 1 || t.chain_big > kMaxRank || t.chain_side < 1 || t.chain_reduce < 1 ||
+      t.chain_reduce > kMaxReduce || t.split_tiles_bits < 0 || t.split_tiles_bits > 24 ||
+      t.split_min_k_bits < 4 || t.split_min_k_bits > 24 || t.ew_chunk < 1 || t.ew_chunk > 64 ||
+      t.direct_chunk < 1 || t.direct_chunk > 64 || !(t.first_batch > 0.0 && t.first_batch <= 1.0)) {
+    set_error("tuning parameter out of range");
+    return TBH_EINVAL;
+  }
+  return 0;
+}
+
+int tbh_get_tuning(tbh_ctx* ctx, tbh_tuning* out) {
+  if (!ctx || !out) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  *out = ctx->tune;
+  return 0;
+}
+
+int tbh_set_tuning(tbh_ctx* ctx, const tbh_tuning* t) {
+  if (!ctx || !t) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  if (int rc = check_tuning(*t)) return rc;
+  if (serving_active(ctx)) {
+    set_error("tuning cannot change while serving");
+    return TBH_EBUSY;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->tune = *t;
+  return 0;
+}
+
+// Coordinate descent over the tuning parameters, each over a short ordered
+// list of values: from the current value, its neighbours in the list are
+// timed (one BATCHED contraction each) and the fastest kept if it beats the
+// current by 2%; passes repeat until nothing improves or the trials run out.
+int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t n, int max_trials,
+                 tbh_tuning* best, double* best_ms) {
+  if (!ctx || (n && !slices) || !best || !best_ms) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  if (n == 0) {
+    set_error("autotune needs slices");
+    return TBH_EINVAL;
+  }
+  if (serving_active(ctx)) {
+    set_error("autotune cannot run while serving");
+    return TBH_EBUSY;
+  }
+  std::vector<uint32_t> ref(n), res(n);
+  tbh_tuning cur;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    cur = ctx->tune;
+  }
+  const tbh_tuning start = cur;
+  auto run = [&](const tbh_tuning& t, double* ms) {
+    {
+      std::lock_guard<std::mutex> g(ctx->mu);
+      ctx->tune = t;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    const int rc = contract_impl(ctx, dtype, slices, n, res.data(), TBH_BATCHED, true);
+    *ms = ms_since(t0);
+    return rc;
+  };
+  auto restore = [&](const tbh_tuning& t) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->tune = t;
+  };
+  double cur_ms = 0;
+  int rc = run(cur, &cur_ms);  // warm-up; its results are the reference
+  if (!rc) {
+    ref = res;
+    rc = run(cur, &cur_ms);
+  }
+  if (rc) {
+    restore(start);
+    return rc;
+  }
+  int trials = 2;
+  struct Param {
+    std::vector<double> values;
+    double tbh_tuning::*d;
+    int32_t tbh_tuning::*i;
+  };
+  const std::vector<Param> params = {
+      {{15, 16, 17, 18, 19}, nullptr, &tbh_tuning::chain_big},
+      {{13, 14, 15, 16, 17, 18}, nullptr, &tbh_tuning::chain_side},
+      {{4, 5, 6, 7, 8, 9}, nullptr, &tbh_tuning::chain_reduce},
+      {{7, 8, 9, 10, 11}, nullptr, &tbh_tuning::split_tiles_bits},
+      {{9, 10, 11, 12}, nullptr, &tbh_tuning::split_min_k_bits},
+      {{1, 2, 4, 8}, nullptr, &tbh_tuning::ew_chunk},
+      {{1, 2, 4, 8}, nullptr, &tbh_tuning::direct_chunk},
+      {{0.05, 0.1, 0.15, 0.25, 0.4}, &tbh_tuning::first_batch, nullptr}};
+  auto get = [](const tbh_tuning& t, const Param& p) { return p.d ? t.*p.d : (double)(t.*p.i); };
+  auto set = [](tbh_tuning& t, const Param& p, double v) {
+    if (p.d)
+      t.*p.d = v;
+    else
+      t.*p.i = (int32_t)v;
+  };
+  for (bool improved = true; improved && trials < max_trials;) {
+    improved = false;
+    for (const Param& p : params) {
+      // neighbours of the current value in the list
+      const double v0 = get(cur, p);
+      size_t at = 0;
+      for (size_t k = 0; k < p.values.size(); k++)
+        if (std::fabs(p.values[k] - v0) < std::fabs(p.values[at] - v0)) at = k;
+      for (int dir : {-1, 1}) {
+        const long k = (long)at + dir;
+        if (k < 0 || k >= (long)p.values.size() || trials >= max_trials) continue;
+        tbh_tuning t = cur;
+        set(t, p, p.values[k]);
+        if (t.chain_side >= t.chain_big || check_tuning(t)) continue;
+        double ms = 0;
+        rc = run(t, &ms);
+        trials++;
+        if (rc) {
+          restore(start);
+          return rc;
+        }
+        if (res != ref) {
+          restore(start);
+          set_error("autotune: results differ between tunings");
+          return TBH_EINVAL;
+        }
+        if (ms < 0.98 * cur_ms) {
+          cur = t;
+          cur_ms = ms;
+          improved = true;
+        }
+      }
+    }
+  }
+  restore(cur);
+  *best = cur;
+  *best_ms = cur_ms;
+  return 0;
 }
 
 int tbh_valu_peak(tbh_ctx* ctx, tbh_dtype dtype, double* tropical_ops_per_s) {

@@ -94,6 +94,7 @@ struct tbh_ctx {
   std::mutex mu;         // one executor call at a time
   tbh_stats stats{};
   int plan_threads = 8;
+  tbh_tuning tune{};   // planner / launch parameters (tbh_set_tuning, tbh_autotune)
   bool profiling = false;
   tbh::Profiler prof;
   std::mutex serve_mu;   // guards `serve` (never held across an execution)
@@ -118,7 +119,7 @@ int convert_slice(const tbh_slice& s, tbh_dtype dtype, bool copy, SliceOwned* ou
 // failed at or before i.
 class PlanStream {
  public:
-  PlanStream(const std::vector<const SliceOwned*>& slices, bool batched, int threads);
+  PlanStream(const std::vector<const SliceOwned*>& slices, const PlanOptions& opt, int threads);
   ~PlanStream();
   const SlicePlan* get(size_t i, std::string* err);
 
@@ -144,5 +145,11 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
                   bool add_r, uint32_t* result);
 int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool batched,
              std::vector<SlicePlan>* plans);
+// the context's tuning as planner options
+PlanOptions plan_options(const tbh_ctx* ctx, bool batched);
+// default tuning (environment overrides TBH_CHAIN="big,side,reduce",
+// TBH_SPLITK="tile_bits,min_k_bits", TBH_EW_CHUNK, TBH_DIRECT_CHUNK,
+// TBH_FIRST_BATCH, read once)
+tbh_tuning default_tuning();
 
 }  // namespace tbh

@@ -115,3 +115,14 @@ def test_serving_entry_points_reject_null_context(tb):
     assert lib.tbh_wait(None, 1, None, C.byref(st)) == _lib.TBH_EINVAL
     assert lib.tbh_cancel(None, 1) == _lib.TBH_EINVAL
     assert b"context" in lib.tbh_last_error()
+
+
+def test_tuning_struct_and_null_context(tb):
+    from tensorbranching_jl_amd import _lib
+    lib = _lib.tbh()
+    assert C.sizeof(_lib.TbhTuning) == 40
+    t = _lib.TbhTuning()
+    assert lib.tbh_get_tuning(None, C.byref(t)) == _lib.TBH_EINVAL
+    assert lib.tbh_set_tuning(None, C.byref(t)) == _lib.TBH_EINVAL
+    ms = C.c_double()
+    assert lib.tbh_autotune(None, 0, None, 0, 4, C.byref(t), C.byref(ms)) == _lib.TBH_EINVAL

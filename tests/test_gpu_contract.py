@@ -146,3 +146,23 @@ def test_slice_directory_on_gpu(tb, orc, backend, tmp_path):
     for mode in ("batched", "plain"):
         assert np.array_equal(backend.contract_slices(back, np.float32, mode), ref)
     assert int(ref.max()) == orc.mis_exact(g)
+
+
+def test_autotune_keeps_results(tb, orc):
+    """The autotuner installs a tuning under which the results are unchanged;
+    out-of-range tunings are refused."""
+    be = tb.HIPBackend(0, pool_bytes=2 << 30)
+    try:
+        g = tb.random_regular_graph(100, 3, seed=0)
+        br = _slices(tb, g, 10)
+        t0 = be.tuning()
+        best, ms = be.autotune(br, np.float32, max_trials=6)
+        assert ms > 0 and best == be.tuning()
+        assert best["chain_side"] < best["chain_big"]
+        assert np.array_equal(be.contract_slices(br, np.float32), orc.contract_slices(br, np.float32))
+        with pytest.raises(tb.TbhError):
+            be.set_tuning(ew_chunk=0)
+        be.set_tuning(**t0)
+        assert be.tuning() == t0
+    finally:
+        be.close()

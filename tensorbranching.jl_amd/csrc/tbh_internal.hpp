@@ -13,7 +13,10 @@ constexpr int kMaxReduce = 12;   // reduced labels of one EW step
 constexpr int kEwBlock = 256;    // threads per EW workgroup
 constexpr int kTileBits = 12;    // a tile has at most 2^12 outputs
 constexpr int kSubBits = 12;     // an LDS sub-tensor has at most 2^12 entries
-constexpr int kLdsElems = (1 << kSubBits) + ((1 << kSubBits) >> 5);  // padded
+#ifndef TBH_LDS_PAD
+#define TBH_LDS_PAD 0  // 1: one pad entry per 32 (bank-conflict padding of staged sub-tensors)
+#endif
+constexpr int kLdsElems = (1 << kSubBits) + (TBH_LDS_PAD ? ((1 << kSubBits) >> 5) : 0);
 
 // GEMM tile shape for an M x N output: tall-skinny outputs (N <= 64, M >=
 // 256) take a 256-row tile so that a narrow N still fills it.
@@ -59,7 +62,7 @@ struct EwDesc {
 };
 
 __host__ __device__ inline uint32_t pos_stride(int8_t p) { return p >= 0 ? (1u << p) : 0u; }
-__host__ __device__ inline uint32_t lds_pad(uint32_t i) { return i + (i >> 5); }
+__host__ __device__ inline uint32_t lds_pad(uint32_t i) { return TBH_LDS_PAD ? i + (i >> 5) : i; }
 
 // One batched max-plus GEMM step (column-major, element strides).
 struct GemmDesc {

@@ -58,6 +58,16 @@ struct KI8 {
 // lgkmcnt and so would make every wait for an LDS or scalar access wait for
 // the data loads in flight too.
 #define TBH_GLOBAL __attribute__((address_space(1)))
+#define TBH_LDS __attribute__((address_space(3)))
+// Sub-tensors are staged with global_load_lds_dword (HBM -> LDS without a
+// register round trip, so every load of a sub-tensor is in flight at once);
+// 0 stages through registers in groups of 8.
+#ifndef TBH_LDS_DMA
+#define TBH_LDS_DMA 1
+#endif
+#if TBH_LDS_DMA && TBH_LDS_PAD
+#error "LDS DMA writes 64 consecutive entries per wave: staging without padding only"
+#endif
 
 template <typename T>
 struct Trop;
@@ -216,7 +226,7 @@ __device__ __forceinline__ void outer_step(KEw d, uint32_t o, EwOuter& r) {
 }
 
 // the sub-tensor of one operand (2^nS elements) into LDS; lds_pad(tid + 256m)
-// = lds_pad(tid) + 264m
+// = lds_pad(tid) + lds_pad(256) m
 template <typename T>
 __device__ __forceinline__ void ew_stage(const TBH_GLOBAL T* __restrict__ X, int nS, uint32_t lo,
                                          const uint32_t (&hS)[4], T* __restrict__ lX) {
@@ -224,6 +234,22 @@ __device__ __forceinline__ void ew_stage(const TBH_GLOBAL T* __restrict__ X, int
   const int n = 1 << nS;
   // recomputed per call: addresses are not held across the caller's loop
   asm volatile("" : "+v"(lo));
+#if TBH_LDS_DMA
+  static_assert(sizeof(T) == 4, "dword LDS DMA");
+  // lane l of the wave writes entry (its wave's first tid) + l + 256 m: the
+  // LDS address is wave-uniform (M0) plus 4 x lane
+  TBH_LDS T* w = (TBH_LDS T*)lX + (tid & ~63);
+  if (n <= 256) {
+    if (tid < n) __builtin_amdgcn_global_load_lds(X + lo, w, 4, 0, 0);
+    return;
+  }
+  const int cnt = n >> 8;  // 2..16, wave-uniform
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    if (m >= cnt) break;
+    __builtin_amdgcn_global_load_lds(X + lo + hi_at(hS, m), w + 256 * m, 4, 0, 0);
+  }
+#else
   const uint32_t pt = lds_pad(tid);
   if (n <= 256) {
     if (tid < n) lX[pt] = X[lo];
@@ -241,8 +267,9 @@ __device__ __forceinline__ void ew_stage(const TBH_GLOBAL T* __restrict__ X, int
       if (g + m < cnt) v[m] = X[lo + hi_at(hS, g + m)];
 #pragma unroll
     for (int m = 0; m < 8; m++)
-      if (g + m < cnt) lX[pt + 264 * (g + m)] = v[m];
+      if (g + m < cnt) lX[pt + lds_pad(256) * (g + m)] = v[m];
   }
+#endif
 }
 
 // One tile of step d at outer offsets off, CNT = max(1, 2^nT / 256) outputs
@@ -301,6 +328,11 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
     }
     ew_stage<T>(A + cA, d->nSA, e.loSA, e.hSA, lA);
     if (HASB) ew_stage<T>(B + cB, d->nSB, e.loSB, e.hSB, lB);
+#if TBH_LDS_DMA
+    // the DMA writes land in LDS when the loads retire (vmcnt), which the
+    // barrier alone does not wait for
+    __builtin_amdgcn_s_waitcnt(0);
+#endif
     __syncthreads();
     if (MA > 0 && active) {
       constexpr int NA = 1 << MA, NB = 16 >> MA;

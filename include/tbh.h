@@ -170,6 +170,9 @@ typedef struct {
   int32_t tile_a_only;      /* EW: tile labels of A and not B               */
   int32_t tile_b_only;      /* EW: tile labels of B and not A               */
   int32_t outer_product;    /* EW: 1 = 4 x 4 outer-product tile layout      */
+  int32_t gemm_store;       /* GEMM: 0 column-major C; C in its consumer's
+                               label order, 1 stored as row quads, 2 staged
+                               through LDS                                  */
   double macs, bytes;       /* executed configurations, algorithmic bytes   */
 } tbh_step_info;
 /* Plans one slice and writes up to cap step records; returns the number of
@@ -183,14 +186,15 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode,
  * chain_big absorbs side tensors of rank <= chain_side, <= chain_reduce
  * labels reduced per step); split_*: GEMM split-K (up to 2^split_tiles_bits
  * tiles per node, K parts >= 2^split_min_k_bits); ew_chunk / direct_chunk:
- * tiles per claim of the tiled / direct EW kernels; first_batch: fraction
- * of the arena region the first batch may use.  Results never depend on
- * them. */
+ * tiles per claim of the tiled / direct EW kernels; ew_per_cu: tiled EW
+ * workgroups per CU in a launch's grid (at most what is resident);
+ * first_batch: fraction of the arena region the first batch may use.
+ * Results never depend on them. */
 typedef struct {
   int32_t chain_big, chain_side, chain_reduce;
   int32_t split_tiles_bits, split_min_k_bits;
   int32_t ew_chunk, direct_chunk;
-  int32_t pad_;
+  int32_t ew_per_cu;
   double first_batch;
 } tbh_tuning;
 int tbh_get_tuning(tbh_ctx* ctx, tbh_tuning* out);

@@ -43,7 +43,7 @@ class TbhStats(C.Structure):
 class TbhTuning(C.Structure):
     _fields_ = [("chain_big", C.c_int32), ("chain_side", C.c_int32), ("chain_reduce", C.c_int32),
                 ("split_tiles_bits", C.c_int32), ("split_min_k_bits", C.c_int32),
-                ("ew_chunk", C.c_int32), ("direct_chunk", C.c_int32), ("pad_", C.c_int32),
+                ("ew_chunk", C.c_int32), ("direct_chunk", C.c_int32), ("ew_per_cu", C.c_int32),
                 ("first_batch", C.c_double)]
 
 
@@ -54,7 +54,7 @@ class TbhStepInfo(C.Structure):
                 ("lanes_a", C.c_int32), ("lanes_b", C.c_int32), ("tile_bits", C.c_int32),
                 ("outer_reduced", C.c_int32), ("direct", C.c_int32), ("tile_a_only", C.c_int32),
                 ("tile_b_only", C.c_int32), ("outer_product", C.c_int32),
-                ("macs", C.c_double), ("bytes", C.c_double)]
+                ("gemm_store", C.c_int32), ("macs", C.c_double), ("bytes", C.c_double)]
 
 
 class TbwSliceView(C.Structure):

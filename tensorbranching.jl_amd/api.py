@@ -279,13 +279,13 @@ class HIPBackend:
         """The context's planner / launch parameters (tbh_tuning) as a dict."""
         t = _lib.TbhTuning()
         check(_lib.tbh().tbh_get_tuning(self.handle, C.byref(t)))
-        return {k: getattr(t, k) for k, _ in t._fields_ if k != "pad_"}
+        return {k: getattr(t, k) for k, _ in t._fields_}
 
     def set_tuning(self, **kw):
         t = _lib.TbhTuning()
         check(_lib.tbh().tbh_get_tuning(self.handle, C.byref(t)))
         for k, v in kw.items():
-            if k not in dict(t._fields_) or k == "pad_":
+            if k not in dict(t._fields_):
                 raise KeyError(k)
             setattr(t, k, v)
         check(_lib.tbh().tbh_set_tuning(self.handle, C.byref(t)))
@@ -299,7 +299,7 @@ class HIPBackend:
         ms = C.c_double()
         check(_lib.tbh().tbh_autotune(self.handle, code, m.ptr, m.n, int(max_trials), C.byref(t),
                                       C.byref(ms)))
-        return {k: getattr(t, k) for k, _ in t._fields_ if k != "pad_"}, ms.value
+        return {k: getattr(t, k) for k, _ in t._fields_}, ms.value
 
     def valu_peak(self, element_type=np.float32):
         """Measured tropical ops/s of the GEMM's VALU instruction mix with

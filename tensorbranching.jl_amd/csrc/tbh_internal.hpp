@@ -12,11 +12,16 @@ constexpr int kMaxRank = 30;     // largest tensor rank (2^30 entries)
 constexpr int kMaxReduce = 12;   // reduced labels of one EW step
 constexpr int kEwBlock = 256;    // threads per EW workgroup
 constexpr int kTileBits = 12;    // a tile has at most 2^12 outputs
+constexpr int kGemmScatter = 40; // GEMM output bit -> position table (rank + tile bits)
+constexpr int kEpiElems = 8192;  // outputs per pass of a GEMM's LDS-staged epilogue
 constexpr int kSubBits = 12;     // an LDS sub-tensor has at most 2^12 entries
-#ifndef TBH_LDS_PAD
-#define TBH_LDS_PAD 0  // 1: one pad entry per 32 (bank-conflict padding of staged sub-tensors)
+// staged sub-tensors hold one pad entry per 2^TBH_LDS_PAD_SHIFT entries (0:
+// none), which moves entries a multiple of that apart into different banks
+#ifndef TBH_LDS_PAD_SHIFT
+#define TBH_LDS_PAD_SHIFT 0
 #endif
-constexpr int kLdsElems = (1 << kSubBits) + (TBH_LDS_PAD ? ((1 << kSubBits) >> 5) : 0);
+constexpr int kLdsElems =
+    (1 << kSubBits) + (TBH_LDS_PAD_SHIFT ? ((1 << kSubBits) >> TBH_LDS_PAD_SHIFT) : 0);
 
 // GEMM tile shape for an M x N output: tall-skinny outputs (N <= 64, M >=
 // 256) take a 256-row tile so that a narrow N still fills it.
@@ -62,7 +67,9 @@ struct EwDesc {
 };
 
 __host__ __device__ inline uint32_t pos_stride(int8_t p) { return p >= 0 ? (1u << p) : 0u; }
-__host__ __device__ inline uint32_t lds_pad(uint32_t i) { return TBH_LDS_PAD ? i + (i >> 5) : i; }
+__host__ __device__ inline uint32_t lds_pad(uint32_t i) {
+  return TBH_LDS_PAD_SHIFT ? i + (i >> TBH_LDS_PAD_SHIFT) : i;
+}
 
 // One batched max-plus GEMM step (column-major, element strides).
 struct GemmDesc {
@@ -72,7 +79,16 @@ struct GemmDesc {
   int64_t sA, sB, sC;
   int32_t tiles_m, tiles_n; // tiles per batch entry
   int32_t shape;            // tile 0: 128 x 128, 1: 256 x 64, 2: 256 x 32
-  int32_t pad_;
+  // scatter: 0 C column-major with ldc; otherwise C has its own label order
+  // (M, N, batch powers of two): bit q of (m, n, batch) -- m bits first,
+  // then n, then batch -- sits at C position pc[q].  1: rows 0..4 sit at
+  // positions 0..4, the tile stores its rows directly; 2: the tile is staged
+  // in LDS and read back in ascending C position (ep), one pass of 2^ep_n
+  // outputs (two passes split on the top column bit of the tile if ep_split)
+  int32_t scatter;
+  int8_t pc[kGemmScatter];  // -1 past the rank
+  int8_t ep[16];            // tile bit b < 32: row bit b; 32 + b: column bit b
+  int32_t ep_n, ep_split;
 };
 
 // Leaf tensor writes (vertex [0,w] or edge [0,0,0,-inf]).

@@ -65,8 +65,8 @@ struct KI8 {
 #ifndef TBH_LDS_DMA
 #define TBH_LDS_DMA 1
 #endif
-#if TBH_LDS_DMA && TBH_LDS_PAD
-#error "LDS DMA writes 64 consecutive entries per wave: staging without padding only"
+#if TBH_LDS_DMA && TBH_LDS_PAD_SHIFT && TBH_LDS_PAD_SHIFT < 6
+#error "LDS DMA writes 64 consecutive entries per wave: pad at most once per 64"
 #endif
 
 template <typename T>
@@ -237,8 +237,9 @@ __device__ __forceinline__ void ew_stage(const TBH_GLOBAL T* __restrict__ X, int
 #if TBH_LDS_DMA
   static_assert(sizeof(T) == 4, "dword LDS DMA");
   // lane l of the wave writes entry (its wave's first tid) + l + 256 m: the
-  // LDS address is wave-uniform (M0) plus 4 x lane
-  TBH_LDS T* w = (TBH_LDS T*)lX + (tid & ~63);
+  // LDS address is wave-uniform (M0) plus 4 x lane, and a run of 64 entries
+  // starting at a multiple of 64 stays contiguous under the padding
+  TBH_LDS T* w = (TBH_LDS T*)lX + lds_pad(tid & ~63);
   if (n <= 256) {
     if (tid < n) __builtin_amdgcn_global_load_lds(X + lo, w, 4, 0, 0);
     return;
@@ -247,7 +248,7 @@ __device__ __forceinline__ void ew_stage(const TBH_GLOBAL T* __restrict__ X, int
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     if (m >= cnt) break;
-    __builtin_amdgcn_global_load_lds(X + lo + hi_at(hS, m), w + 256 * m, 4, 0, 0);
+    __builtin_amdgcn_global_load_lds(X + lo + hi_at(hS, m), w + lds_pad(256) * m, 4, 0, 0);
   }
 #else
   const uint32_t pt = lds_pad(tid);
@@ -709,14 +710,155 @@ struct GemmShape<2> {
   static constexpr int BM = 256, BN = 32, TN = 4;
 };
 
+// Offset of the output bits x (bit b is GEMM output bit q0 + b) in a C whose
+// bit q sits at position pc[q] (-1: no position; only rows / columns past M /
+// N, which are not written, reach those).
+__device__ __forceinline__ int64_t scatter_bits(const int8_t* __restrict__ pc, int q0, uint32_t x) {
+  int64_t r = 0;
+  for (int b = 0; x != 0 && q0 + b < kGemmScatter; b++, x >>= 1) {
+    const int p = pc[q0 + b];
+    if ((x & 1) && p >= 0) r += (int64_t)1 << p;
+  }
+  return r;
+}
+
+// Output tile of a GEMM whose C has its own label order (scatter 2): the
+// tile goes to LDS, then the threads read it back in ascending C position
+// (enumeration bit k of output e = tid + 256 r is tile bit ep[k]), so that
+// consecutive lanes store consecutive addresses.  In LDS, output (m, n) of a
+// pass (n without the split bit) sits at (m ^ 4 (n & s)) + 2^vm n: quads of
+// rows stay contiguous and 16-byte aligned, and the XOR spreads the columns
+// of one row quad over the banks.  Every output of a pass is a valid one: M
+// and N are powers of two, so min(M, BM) rows and min(N, BN) columns of the
+// tile exist.  P is the pass: with a split, pass P holds the columns whose
+// top tile bit is P (the thread's maxima j >= 4 when P = 1), else P = 0
+// holds the whole tile.
+template <typename T, int SHAPE, int P>
+__device__ __forceinline__ void gemm_epilogue_pass(TBH_GLOBAL T* __restrict__ C,
+                                                   const T (&acc)[8][GemmShape<SHAPE>::TN],
+                                                   int m0, int n0, int M, int N, int64_t base,
+                                                   const int8_t* __restrict__ pc, int ma,
+                                                   const GemmDesc* __restrict__ d,
+                                                   T* __restrict__ st) {
+  constexpr int BM = GemmShape<SHAPE>::BM, BN = GemmShape<SHAPE>::BN, TN = GemmShape<SHAPE>::TN;
+  constexpr int LBM = BM == 256 ? 8 : 7;
+  using V = typename Vec4<T>::type;
+  const int tid = threadIdx.x;
+  const int tx = tid % (BM / 8), ty = tid / (BM / 8);
+  const int vm = ma < LBM ? ma : LBM;
+  const bool split = d->ep_split != 0;
+  const int nbits = d->ep_n;  // 12 .. 13
+  // the XOR mask: row bits 2 .. vm-1 (at most 5 of them)
+  const uint32_t smask = (1u << ((vm - 2) < 5 ? (vm - 2) : 5)) - 1;
+#pragma unroll
+  for (int j = 0; j < TN; j++) {
+    const int nl = j < 4 ? ty * 4 + j : BN / 2 + ty * 4 + (j - 4);
+    const bool mine = split ? ((j >= 4) == (P == 1)) : true;
+    if (mine && n0 + nl < N) {
+      const uint32_t ns = split && j >= 4 ? nl - BN / 2 : nl;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const uint32_t ml = h * (BM / 2) + tx * 4;
+        if (m0 + (int)ml < M) {
+          const uint32_t phys = (ml ^ ((ns & smask) << 2)) + (ns << vm);
+          T x0 = Trop<T>::canon(acc[4 * h][j]), x1 = Trop<T>::canon(acc[4 * h + 1][j]),
+            x2 = Trop<T>::canon(acc[4 * h + 2][j]), x3 = Trop<T>::canon(acc[4 * h + 3][j]);
+          // opaque: a quad gathered straight from the maxima array gets
+          // vectorised into a load from it, which keeps the whole array out
+          // of registers (in scratch) through the k loop
+          asm volatile("" : "+v"(x0), "+v"(x1), "+v"(x2), "+v"(x3));
+          *reinterpret_cast<V*>(st + phys) = V{x0, x1, x2, x3};
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // enumeration bit k as its share of the LDS index (xk: XORed part, ak:
+  // added part; row bit b: xk = 2^b; column bit b: xk = 4 (2^b & s), ak =
+  // 2^(vm+b)) and of the C offset (gk), decoded once (uniform)
+  uint32_t xk[13], ak[13], gk[13];
+#pragma unroll
+  for (int k = 0; k < 13; k++) {
+    xk[k] = ak[k] = gk[k] = 0;
+    if (k < nbits) {
+      const int b = d->ep[k];
+      if (b < 32) {
+        xk[k] = 1u << b;
+        gk[k] = 1u << pc[b];
+      } else {
+        xk[k] = ((1u << (b - 32)) & smask) << 2;
+        ak[k] = (1u << (b - 32)) << vm;
+        gk[k] = 1u << pc[ma + b - 32];
+      }
+    }
+  }
+  // enumeration bits 0 and 1 are C positions 0 and 1 (ep lists the pass's
+  // bits by C position, and C's lowest positions are pass bits): a thread
+  // owns 4 consecutive outputs, one aligned 16-byte store; bits 2..9 are
+  // the thread, bits 10.. the iteration r
+  uint32_t xt = 0, at = 0, gt = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const bool on = (tid >> k) & 1;
+    xt ^= on ? xk[2 + k] : 0u;
+    at += on ? ak[2 + k] : 0u;
+    gt += on ? gk[2 + k] : 0u;
+  }
+  uint32_t li[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    li[i] = ((i & 1) ? xk[0] : 0u) ^ ((i & 2) ? xk[1] : 0u);
+  const uint32_t a1 = ak[0], a2 = ak[1];
+  TBH_GLOBAL T* __restrict__ Ct = C + base;
+  const int R = 1 << (nbits - 10);  // 4 .. 8 per pass
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    if (r >= R) break;
+    uint32_t xr = xt, ar = at, gr = gt;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      if ((r >> k) & 1) {
+        xr ^= xk[10 + k];
+        ar += ak[10 + k];
+        gr += gk[10 + k];
+      }
+    V q;
+    q.x = st[(xr ^ li[0]) + ar];
+    q.y = st[(xr ^ li[1]) + ar + a1];
+    q.z = st[(xr ^ li[2]) + ar + a2];
+    q.w = st[(xr ^ li[3]) + ar + a1 + a2];
+    *reinterpret_cast<TBH_GLOBAL V*>(Ct + gr) = q;
+  }
+  __syncthreads();
+}
+
+template <typename T, int SHAPE>
+__device__ __forceinline__ void gemm_epilogue_lds(TBH_GLOBAL T* __restrict__ C,
+                                                  const T (&acc)[8][GemmShape<SHAPE>::TN], int m0,
+                                                  int n0, int M, int N,
+                                                  const int8_t* __restrict__ pc, int ma,
+                                                  const GemmDesc* __restrict__ d,
+                                                  T* __restrict__ st) {
+  constexpr int BN = GemmShape<SHAPE>::BN;
+  constexpr int LBN = BN == 128 ? 7 : BN == 64 ? 6 : 5;
+  const int64_t base = scatter_bits(pc, 0, m0) + scatter_bits(pc, ma, n0);
+  gemm_epilogue_pass<T, SHAPE, 0>(C, acc, m0, n0, M, N, base, pc, ma, d, st);
+  if (GemmShape<SHAPE>::TN == 8 && d->ep_split)
+    gemm_epilogue_pass<T, SHAPE, 1>(C, acc, m0, n0, M, N,
+                                    base + scatter_bits(pc, ma + LBN - 1, 1), pc, ma, d, st);
+}
+
 // One BM x BN tile of one batch entry.  VEC: A, B 16-byte aligned with lda,
-// ldb, M, K multiples of 4.
+// ldb, M, K multiples of 4.  pc != nullptr: C is written in its own label
+// order (scatter_bits; M a power of two, ma = log2 M), else column-major with
+// ldc.
 template <typename T, bool VEC, int SHAPE>
 __device__ __forceinline__ void gemm_tile(
     const TBH_GLOBAL T* __restrict__ A, const TBH_GLOBAL T* __restrict__ B,
     TBH_GLOBAL T* __restrict__ C, int M, int N, int K, int64_t lda, int64_t ldb, int64_t ldc,
-    int m0, int n0, T (*As)[BK][GemmShape<SHAPE>::BM],
-    T (*Bs)[BK][GemmShape<SHAPE>::BN + BPAD]) {
+    int m0, int n0, const int8_t* __restrict__ pc, int ma, int nb, const GemmDesc* __restrict__ d,
+    T (*As)[BK][GemmShape<SHAPE>::BM], T (*Bs)[BK][GemmShape<SHAPE>::BN + BPAD],
+    T* __restrict__ smem) {
   constexpr int BM = GemmShape<SHAPE>::BM, BN = GemmShape<SHAPE>::BN, TN = GemmShape<SHAPE>::TN;
   constexpr int TX = BM / 8;                    // threads along M
   constexpr int NVA = BM * BK / 4 / 256;        // 4-vectors of A per thread
@@ -847,6 +989,37 @@ __device__ __forceinline__ void gemm_tile(
     __syncthreads();
     buf ^= 1;
   }
+  if (pc && d->scatter == 2) {
+    gemm_epilogue_lds<T, SHAPE>(C, acc, m0, n0, M, N, pc, ma, d, smem);
+    return;
+  }
+  if (pc) {
+    // row m0 + h BM/2 + tx 4 + i: its bits are disjoint parts (m0 a multiple
+    // of BM, tx 4 < BM/2, i < 4), so its offset is a sum of per-part offsets;
+    // the same for columns.  Row bits 0 and 1 sit at C positions 0 and 1 (pc
+    // maps them so), every other bit at a position >= 2: the four rows
+    // tx 4 .. +3 are one aligned quad.
+    const int64_t rb = scatter_bits(pc, 0, m0) + scatter_bits(pc, 0, tx * 4);
+    const int64_t rh = scatter_bits(pc, 0, BM / 2);
+    const int64_t cb = scatter_bits(pc, ma, n0) + scatter_bits(pc, ma, ty * 4);
+    const int64_t ch = scatter_bits(pc, ma, BN / 2), c1 = scatter_bits(pc, ma, 1),
+                  c2 = scatter_bits(pc, ma, 2);
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+      const int n = n0 + (j < 4 ? ty * 4 + j : BN / 2 + ty * 4 + (j - 4));
+      if (n >= N) continue;
+      const int64_t co = cb + (j >= 4 ? ch : 0) + ((j & 1) ? c1 : 0) + ((j & 2) ? c2 : 0);
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int m = m0 + h * (BM / 2) + tx * 4;
+        if (m < M)
+          *reinterpret_cast<TBH_GLOBAL V*>(C + co + rb + (h ? rh : 0)) =
+              V{Trop<T>::canon(acc[4 * h][j]), Trop<T>::canon(acc[4 * h + 1][j]),
+                Trop<T>::canon(acc[4 * h + 2][j]), Trop<T>::canon(acc[4 * h + 3][j])};
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < TN; j++) {
     const int n = n0 + (j < 4 ? ty * 4 + j : BN / 2 + ty * 4 + (j - 4));
@@ -877,8 +1050,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBH_GEMM_WA
 tbh_gemm_kernel(const GemmDesc* __restrict__ ops, const int64_t* __restrict__ prefix, int nops,
                 int64_t total) {
   constexpr int BM = GemmShape<SHAPE>::BM, BN = GemmShape<SHAPE>::BN;
-  __shared__ T As[2][BK][BM];
-  __shared__ T Bs[2][BK][BN + BPAD];
+  // A and B tiles (double-buffered), and after the k loop the output tile
+  // of a scattered epilogue (up to kEpiElems entries)
+  constexpr int NA = 2 * BK * BM, NB = 2 * BK * (BN + BPAD);
+  __shared__ __align__(16) T smem[NA + NB > kEpiElems ? NA + NB : kEpiElems];
+  auto As = reinterpret_cast<T(*)[BK][BM]>(smem);
+  auto Bs = reinterpret_cast<T(*)[BK][BN + BPAD]>(smem + NA);
   for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
     int lo = 0, hi = nops;
     while (hi - lo > 1) {
@@ -896,9 +1073,13 @@ tbh_gemm_kernel(const GemmDesc* __restrict__ ops, const int64_t* __restrict__ pr
     const int tm = (int)(rem % d->tiles_m), tn = (int)(rem / d->tiles_m);
     const TBH_GLOBAL T* A = reinterpret_cast<const TBH_GLOBAL T*>(d->a) + t * d->sA;
     const TBH_GLOBAL T* B = reinterpret_cast<const TBH_GLOBAL T*>(d->b) + t * d->sB;
-    TBH_GLOBAL T* C = reinterpret_cast<TBH_GLOBAL T*>(d->c) + t * d->sC;
+    const int8_t* pc = d->scatter ? d->pc : nullptr;
+    const int ma = pc ? __builtin_ctz((uint32_t)d->M) : 0;
+    const int nb = pc ? __builtin_ctz((uint32_t)d->N) : 0;
+    TBH_GLOBAL T* C = reinterpret_cast<TBH_GLOBAL T*>(d->c) +
+                      (pc ? scatter_bits(pc, ma + nb, (uint32_t)t) : t * d->sC);
     gemm_tile<T, VEC, SHAPE>(A, B, C, d->M, d->N, d->K, d->lda, d->ldb, d->ldc, tm * BM, tn * BN,
-                             As, Bs);
+                             pc, ma, nb, d, As, Bs, smem);
     __syncthreads();
   }
 }
@@ -984,6 +1165,14 @@ hipError_t launch_ew(int dtype, const EwDesc* ops, const int64_t* prefix, int no
   else
     tbh_ew_kernel<int32_t><<<grid, kEwBlock, 0, s>>>(ops, prefix, nops, total, chunk);
   return hipGetLastError();
+}
+
+int ew_blocks_per_cu(int dtype) {
+  int n = 0;
+  const hipError_t err =
+      dtype == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tbh_ew_kernel<float>, kEwBlock, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tbh_ew_kernel<int32_t>, kEwBlock, 0);
+  return err == hipSuccess && n > 0 ? n : 4;
 }
 
 hipError_t launch_ew_direct(int dtype, const EwDesc* ops, const int64_t* prefix, int nops,

@@ -372,6 +372,40 @@ bool gemm_shaped(const PlanOptions& opt, int na, int nb, int nk) {
          K >= opt.gemm_min_k;
 }
 
+// How a GEMM writing the labels G (m bits, n bits, batch bits) stores into a
+// tensor of layout L (GemmDesc::scatter), or 0 when a permute step is
+// needed: every one of L's lowest gemm_scatter_bits positions must be a tile
+// bit of one epilogue pass, so each tile writes whole runs of that many
+// elements.  1 when L's positions 0 and 1 are rows 0 and 1 (the tile's own
+// row quads are 16-byte runs of C), else 2 (staged through LDS, ep the
+// pass's tile bits in ascending position of L).
+int gemm_scatter_mode(const PlanOptions& opt, const std::vector<int32_t>& L,
+                      const std::vector<int32_t>& G, int na, int nb, POp* g) {
+  if (opt.gemm_scatter_bits < 0 || L.size() != G.size() || (int)G.size() > kGemmScatter - 8 ||
+      na < 2)
+    return 0;
+  const int shape = gemm_shape((int64_t)1 << na, (int64_t)1 << nb);
+  const int bm = __builtin_ctz(gemm_bm(shape)), bn = __builtin_ctz(gemm_bn(shape));
+  const int vm = std::min(na, bm), vn = std::min(nb, bn);
+  const int split = vm + vn > 13 ? 1 : 0;  // an LDS pass holds kEpiElems = 2^13 outputs
+  // the pass's tile bits: rows below vm, columns below vn (less the top one)
+  std::vector<std::pair<int, int>> bits;  // (position in L, id)
+  for (int b = 0; b < vm; b++) bits.push_back({pos_in(L, G[b]), b});
+  for (int b = 0; b < vn - split; b++) bits.push_back({pos_in(L, G[na + b]), 32 + b});
+  std::sort(bits.begin(), bits.end());
+  // (positions 0 and 1 at least: the LDS-staged epilogue stores quads)
+  const int low = std::min<int>(std::max(opt.gemm_scatter_bits, 2), (int)L.size());
+  for (int p = 0; p < low; p++)
+    if (p >= (int)bits.size() || bits[p].first != p) return 0;
+  if ((int)bits.size() < 10) return 0;
+  g->ep_n = (int)bits.size();
+  g->ep_split = split;
+  for (size_t k = 0; k < bits.size(); k++) g->ep[k] = (int8_t)bits[k].second;
+  // rows 0 and 1 at positions 0 and 1: each thread's row quads are 16-byte
+  // runs of C, stored directly (L2 completes the lines of a tile)
+  return bits[0].second == 0 && bits[1].second == 1 ? 1 : 2;
+}
+
 // Split-K for a GEMM node (batched plans): a node with few 128 x 128 tiles
 // and a long K loop keeps its s highest K labels as batch labels of a
 // temporary, reduced afterwards by an EW step, so that its tiles spread over
@@ -506,6 +540,46 @@ int verify_plan(const SlicePlan& P, std::string* err) {
           ec != ((int64_t)1 << rc))
         return bad("GEMM extents");
       if (op.M % 4 || op.K % 4) return bad("GEMM vector alignment");
+      if (op.scatter) {
+        // pc: a bijection of the output bits onto C's positions, -1 past
+        // them; scatter 1: rows 0 and 1 at positions 0 and 1 (quad stores);
+        // scatter 2: ep lists each valid tile bit of a pass once, at most
+        // 2^13 outputs per pass
+        if (op.scatter < 1 || op.scatter > 2) return bad("GEMM scatter mode");
+        const int na = __builtin_ctz((uint32_t)op.M), nb = __builtin_ctz((uint32_t)op.N);
+        if (op.M != (1 << na) || op.N != (1 << nb)) return bad("GEMM scatter extents");
+        if (op.scatter == 1 && (op.pc[0] != 0 || op.pc[1] != 1))
+          return bad("GEMM output map: rows 0, 1");
+        if (op.scatter == 2) {
+          const int shape = gemm_shape(op.M, op.N);
+          const int vm = std::min(na, __builtin_ctz(gemm_bm(shape)));
+          const int vn = std::min(nb, __builtin_ctz(gemm_bn(shape)));
+          if (vm < 2 || op.ep_split != (vm + vn > 13 ? 1 : 0) || op.ep_n != vm + vn - op.ep_split ||
+              op.ep_n < 10 || op.ep_n > 13)
+            return bad("GEMM epilogue passes");
+          uint64_t seen = 0;
+          for (int k = 0; k < op.ep_n; k++) {
+            const int b = op.ep[k];
+            const bool ok = (b >= 0 && b < vm) || (b >= 32 && b - 32 < vn - op.ep_split);
+            if (!ok || ((seen >> b) & 1)) return bad("GEMM epilogue bits");
+            seen |= 1ull << b;
+            // the first two at C positions 0 and 1 (quad stores)
+            const int q = b < 32 ? b : na + b - 32;
+            if (k < 2 && op.pc[q] != k) return bad("GEMM epilogue quads");
+          }
+        }
+        uint64_t used = 0;
+        for (int q = 0; q < kGemmScatter; q++) {
+          const int p = op.pc[q];
+          if (q >= rc) {
+            if (p != -1) return bad("GEMM output map past the rank");
+          } else if (p < 0 || p >= rc || ((used >> p) & 1)) {
+            return bad("GEMM output map");
+          } else {
+            used |= 1ull << p;
+          }
+        }
+      }
     }
   }
   return 0;
@@ -795,7 +869,16 @@ int plan_slice(const SliceIn& s_in, const PlanOptions& opt_in, SlicePlan* out, s
       bool g = !c.priv && gemm_shaped(opt, na, nb, nk);
       is_gemm[j] = g;
       if (g) {
-        auto sa = sorted_key(c.a), sb = sorted_key(c.b), sk = sorted_key(c.k), st = sorted_key(c.t);
+        // the order of the M, N and batch labels is free: with a layout
+        // request from the consumer, follow it (the output then needs no
+        // permute, or one the GEMM's own stores can do)
+        auto by_req = [&](std::vector<int32_t> v) {
+          if (!has_req[o] || opt.gemm_scatter_bits < 0) return sorted_key(v);
+          std::sort(v.begin(), v.end(),
+                    [&](int32_t x, int32_t y) { return pos_in(req[o], x) < pos_in(req[o], y); });
+          return v;
+        };
+        auto sa = by_req(c.a), sb = by_req(c.b), sk = sorted_key(c.k), st = by_req(c.t);
         // split-K: the last ks K labels (the latest eliminated stay lowest)
         // become batch labels between the outputs and t
         const int ks = gemm_split(opt, na, nb, nk, (int)st.size());
@@ -823,9 +906,23 @@ int plan_slice(const SliceIn& s_in, const PlanOptions& opt_in, SlicePlan* out, s
       Cls& c = cls[j];
       if (is_gemm[j]) {
         int dst = o;
-        if (B.layout[o] != gemm_out[j]) dst = B.add_tensor(gemm_out[j]);
-        B.gemm(c.ca, c.cb, dst, (int)c.a.size(), (int)c.b.size(), (int)c.k.size() - split[j],
-               (int)c.t.size() + split[j]);
+        const int na = (int)c.a.size(), nb = (int)c.b.size();
+        POp epi;
+        const int scatter = B.layout[o] != gemm_out[j] && split[j] == 0
+                                ? gemm_scatter_mode(opt, B.layout[o], gemm_out[j], na, nb, &epi)
+                                : 0;
+        if (B.layout[o] != gemm_out[j] && !scatter) dst = B.add_tensor(gemm_out[j]);
+        B.gemm(c.ca, c.cb, dst, na, nb, (int)c.k.size() - split[j], (int)c.t.size() + split[j]);
+        if (scatter) {
+          POp& g = P.ops.back();
+          g.scatter = scatter;
+          std::fill(std::begin(g.pc), std::end(g.pc), (int8_t)-1);
+          for (size_t q = 0; q < gemm_out[j].size(); q++)
+            g.pc[q] = (int8_t)pos_in(B.layout[o], gemm_out[j][q]);
+          std::memcpy(g.ep, epi.ep, sizeof g.ep);
+          g.ep_n = epi.ep_n;
+          g.ep_split = epi.ep_split;
+        }
         if (dst != o && B.ew(dst, -1, o)) return -1;
       } else {
         if (B.ew(s.tree[2 * j], s.tree[2 * j + 1], o)) return -1;

@@ -43,6 +43,10 @@ struct POp {
   EwDesc ew{};
   // GEMM
   int32_t M = 0, N = 0, K = 0, batch = 0;
+  int scatter = 0;               // GemmDesc::scatter: C in its own label order, GEMM
+  int8_t pc[kGemmScatter] = {};  // output bit q (m, then n, then batch bits) at pc[q]
+  int8_t ep[16] = {};            // scatter 2: a pass's tile bits in ascending C position
+  int ep_n = 0, ep_split = 0;
   double macs = 0;     // executed (output x reduced) configurations
   double bytes = 0;    // algorithmic bytes: |A| + |B| + |C| elements x 4
 };
@@ -80,6 +84,11 @@ struct PlanOptions {
   // tensor <= chain_side, at most chain_reduce labels reduced per step
   bool reassociate = true;
   int chain_big = 17, chain_side = 16, chain_reduce = 7;
+  // a GEMM node writes its output straight in the layout its consumer asks
+  // for (instead of a permute step after it) when the output's lowest
+  // gemm_scatter_bits positions are row or column bits of one tile, so each
+  // tile writes whole runs of 2^gemm_scatter_bits elements (< 0: never)
+  int gemm_scatter_bits = 5;
 };
 
 // Returns 0 or sets *err and returns -1.

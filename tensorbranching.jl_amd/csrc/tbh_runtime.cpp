@@ -231,6 +231,7 @@ tbh_tuning default_tuning() {
     d.split_min_k_bits = o.split_min_k_bits;
     d.ew_chunk = 2;
     d.direct_chunk = 4;
+    d.ew_per_cu = 4;
     d.first_batch = 0.15;
     if (const char* e = std::getenv("TBH_CHAIN"))
       std::sscanf(e, "%d,%d,%d", &d.chain_big, &d.chain_side, &d.chain_reduce);
@@ -238,6 +239,8 @@ tbh_tuning default_tuning() {
       std::sscanf(e, "%d,%d", &d.split_tiles_bits, &d.split_min_k_bits);
     if (const char* e = std::getenv("TBH_EW_CHUNK")) d.ew_chunk = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("TBH_DIRECT_CHUNK")) d.direct_chunk = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("TBH_EW_PER_CU"))
+      d.ew_per_cu = std::min(8, std::max(1, std::atoi(e)));
     if (const char* e = std::getenv("TBH_FIRST_BATCH"))
       d.first_batch = std::min(1.0, std::max(0.01, std::atof(e)));
     return d;
@@ -254,6 +257,11 @@ PlanOptions plan_options(const tbh_ctx* ctx, bool batched) {
   o.chain_reduce = t.chain_reduce;
   o.split_tiles_bits = t.split_tiles_bits;
   o.split_min_k_bits = t.split_min_k_bits;
+  static const int scatter_bits = [] {
+    const char* e = std::getenv("TBH_GEMM_SCATTER");
+    return e ? std::atoi(e) : PlanOptions().gemm_scatter_bits;
+  }();
+  o.gemm_scatter_bits = scatter_bits;
   return o;
 }
 
@@ -383,7 +391,11 @@ GemmDesc make_gemm(const POp& op, const std::vector<PTensor>& T, uint64_t base) 
   d.sB = (int64_t)op.K * op.N;
   d.sC = (int64_t)op.M * op.N;
   d.shape = gemm_shape(op.M, op.N);
-  d.pad_ = 0;
+  d.scatter = op.scatter;
+  std::memcpy(d.pc, op.pc, sizeof d.pc);
+  std::memcpy(d.ep, op.ep, sizeof d.ep);
+  d.ep_n = op.ep_n;
+  d.ep_split = op.ep_split;
   d.tiles_m = (op.M + gemm_bm(d.shape) - 1) / gemm_bm(d.shape);
   d.tiles_n = (op.N + gemm_bn(d.shape) - 1) / gemm_bn(d.shape);
   return d;
@@ -391,10 +403,13 @@ GemmDesc make_gemm(const POp& op, const std::vector<PTensor>& T, uint64_t base) 
 
 int64_t gemm_tiles(const GemmDesc& d) { return (int64_t)d.tiles_m * d.tiles_n * d.batch; }
 
-// EW workgroups: one tile chunk each, at most 4 resident per CU (LDS 33 KB)
-int ew_grid(tbh_ctx* ctx, int64_t total, int chunk) {
+// EW workgroups: one tile chunk each, ew_per_cu per CU but no more than are
+// resident at once (5: 32 KB of LDS each).  4 measured faster than 5 on the
+// 3-regular n=300 slices.
+int ew_grid(tbh_ctx* ctx, int dtype, int64_t total, int chunk) {
   int64_t blocks = (total + chunk - 1) / chunk;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)ctx->num_cus * 4));
+  const int per_cu = std::min(ctx->tune.ew_per_cu, ctx->ew_occ[dtype ? 1 : 0]);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)ctx->num_cus * per_cu));
 }
 
 int gemm_grid(tbh_ctx* ctx, int64_t total) {
@@ -674,7 +689,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
                                 direct_chunk(ctx, total), ctx->stream);
       if (k == 1)
         return launch_ew(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
-                         ew_grid(ctx, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
+                         ew_grid(ctx, dtype, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
                          ctx->stream);
       return launch_gemm(dtype, true, k - 2, reinterpret_cast<const GemmDesc*>(descs), prefix,
                          nops, total, gemm_grid(ctx, total), ctx->stream);
@@ -832,7 +847,7 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
       else
         err = launch_ew(dtype, (const EwDesc*)(dbase + o_op[k]),
                         (const int64_t*)(dbase + o_pre[k]), 1, tiles[k],
-                        ew_grid(ctx, tiles[k], ew_chunk(ctx, tiles[k])), ew_chunk(ctx, tiles[k]),
+                        ew_grid(ctx, dtype, tiles[k], ew_chunk(ctx, tiles[k])), ew_chunk(ctx, tiles[k]),
                         ctx->stream);
       ctx->stats.ew_ops++;
       ctx->stats.ew_ops_algo_bytes += op.bytes;
@@ -895,6 +910,7 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
   ctx->num_cus = prop.multiProcessorCount;
+  for (int t = 0; t < 2; t++) ctx->ew_occ[t] = ew_blocks_per_cu(t);
   size_t fr = 0, tot = 0;
   HIP_TRY(hipMemGetInfo(&fr, &tot));
   if (pool_bytes == 0) pool_bytes = (size_t)(fr * 0.8);
@@ -1082,7 +1098,7 @@ int tbh_maxplus_gemm(tbh_ctx* ctx, tbh_dtype dtype, int64_t M, int64_t N, int64_
   d.sB = strideB;
   d.sC = strideC;
   d.shape = gemm_shape(M, N);
-  d.pad_ = 0;
+  d.scatter = 0;
   d.tiles_m = (int32_t)((M + gemm_bm(d.shape) - 1) / gemm_bm(d.shape));
   d.tiles_n = (int32_t)((N + gemm_bn(d.shape) - 1) / gemm_bn(d.shape));
   const int64_t total = gemm_tiles(d);
@@ -1240,6 +1256,7 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode, tbh_s
         s.tile_b_only += op.ew.tB[k] >= 0 && op.ew.tA[k] < 0;
       }
     }
+    if (op.kind == kGemm) s.gemm_store = op.scatter;
     s.macs = op.macs;
     s.bytes = op.bytes;
   }
@@ -1255,7 +1272,8 @@ static int check_tuning(const tbh_tuning& t) {
   if (t.chain_big < 1 || t.chain_big > kMaxRank || t.chain_side < 1 || t.chain_reduce < 1 ||
       t.chain_reduce > kMaxReduce || t.split_tiles_bits < 0 || t.split_tiles_bits > 24 ||
       t.split_min_k_bits < 4 || t.split_min_k_bits > 24 || t.ew_chunk < 1 || t.ew_chunk > 64 ||
-      t.direct_chunk < 1 || t.direct_chunk > 64 || !(t.first_batch > 0.0 && t.first_batch <= 1.0)) {
+      t.direct_chunk < 1 || t.direct_chunk > 64 || t.ew_per_cu < 1 || t.ew_per_cu > 8 ||
+      !(t.first_batch > 0.0 && t.first_batch <= 1.0)) {
     set_error("tuning parameter out of range");
     return TBH_EINVAL;
   }
@@ -1350,6 +1368,7 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
       {{9, 10, 11, 12}, nullptr, &tbh_tuning::split_min_k_bits},
       {{1, 2, 4, 8}, nullptr, &tbh_tuning::ew_chunk},
       {{1, 2, 4, 8}, nullptr, &tbh_tuning::direct_chunk},
+      {{2, 3, 4, 5}, nullptr, &tbh_tuning::ew_per_cu},
       {{0.05, 0.1, 0.15, 0.25, 0.4}, &tbh_tuning::first_batch, nullptr}};
   auto get = [](const tbh_tuning& t, const Param& p) { return p.d ? t.*p.d : (double)(t.*p.i); };
   auto set = [](tbh_tuning& t, const Param& p, double v) {

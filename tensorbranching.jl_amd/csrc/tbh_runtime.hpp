@@ -32,6 +32,8 @@ hipError_t launch_gemm(int dtype, bool vec, int shape, const GemmDesc* ops, cons
 hipError_t launch_leaves(int dtype, const LeafDesc* leaves, int64_t n, hipStream_t s);
 hipError_t launch_epilogue(int dtype, const EpiDesc* e, int64_t n, void* out, hipStream_t s);
 hipError_t launch_valu_probe(int dtype, void* out, int blocks, int iters, hipStream_t s);
+// resident tiled-EW workgroups per CU (LDS and registers), from the runtime
+int ew_blocks_per_cu(int dtype);
 
 // First-fit allocator over one device allocation (offsets, 256-byte units).
 class Pool {
@@ -86,6 +88,7 @@ struct Profiler {
 struct tbh_ctx {
   int device = 0;
   int num_cus = 256;
+  int ew_occ[2] = {4, 4};  // resident tiled-EW workgroups per CU, per element type
   hipStream_t stream = nullptr;
   char* base = nullptr;  // device pool
   tbh::Pool pool;

@@ -91,6 +91,22 @@ def test_plan_steps_cover_every_node(tb):
     assert [s["level"] for s in plain] == list(range(1, len(plain) + 1))
 
 
+def test_gemm_output_store_modes(tb):
+    """Batched plans write a GEMM node's output straight into the label order
+    its consumer asks for (as row quads, or staged through LDS) instead of a
+    permute step after it; plain plans (OMEinsum's lowering) never do."""
+    g = tb.random_regular_graph(200, 3, seed=3)
+    br = tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=22, order_trials=30), seed=3)
+    stores = {"batched": set(), "plain": set()}
+    for b in br:
+        if b.code is None:
+            continue
+        for mode in stores:
+            stores[mode] |= {s["gemm_store"] for s in tb.plan_steps(b, mode=mode) if s["kind"] == 1}
+    assert stores["batched"] == {0, 1, 2}
+    assert stores["plain"] == {0}
+
+
 def test_footprint_bounds_reference_model(tb):
     """The batched arena holds at least the largest tensor and is of the
     order of the reference's peak-memory model."""

@@ -134,6 +134,24 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
         assert np.array_equal(backend.contract_slices(br, dtype, mode), ref), mode
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.int32])
+def test_gemm_output_stores_bit_exact(tb, orc, backend, dtype):
+    """GEMM nodes that write their output in the consumer's label order (row
+    quads, or an LDS-staged tile read back along the output's lowest
+    positions, in one or two passes): every slice's optimum equals the
+    oracle's."""
+    g = tb.random_regular_graph(200, 3, seed=3)
+    w = None
+    if dtype == np.int32:
+        w = np.random.default_rng(3).integers(1, 101, g.nv).astype(np.int32)
+    br = tb.slice_graph(g, w, tb.ContractionTreeSlicer(sc_target=22, order_trials=30), seed=3)
+    stores = {s["gemm_store"] for b in br if b.code is not None
+              for s in tb.plan_steps(b, dtype) if s["kind"] == 1}
+    assert stores == {0, 1, 2}
+    ref = orc.contract_slices(br, dtype)
+    assert np.array_equal(backend.contract_slices(br, dtype, "batched"), ref)
+
+
 def test_slice_directory_on_gpu(tb, orc, backend, tmp_path):
     """A slice directory in the reference's on-disk format (src/io.jl),
     loaded with load_all_finished, contracts on the GPU to the oracle's

@@ -65,6 +65,11 @@ struct KI8 {
 #ifndef TBH_LDS_DMA
 #define TBH_LDS_DMA 1
 #endif
+// 16-byte DMA (global_load_lds_dwordx4) where a sub-tensor's lowest bits
+// are contiguous in the operand
+#ifndef TBH_LDS_X4
+#define TBH_LDS_X4 1
+#endif
 #if TBH_LDS_DMA && TBH_LDS_PAD_SHIFT && TBH_LDS_PAD_SHIFT < 6
 #error "LDS DMA writes 64 consecutive entries per wave: pad at most once per 64"
 #endif
@@ -142,6 +147,10 @@ __device__ __forceinline__ uint32_t lo_dep(uint32_t tid, KI8 pos) {
 struct EwThread {  // per-step constants of one thread
   uint32_t loSA, loSB, loTA, loTB, loTC;
   uint32_t hSA[4], hSB[4], hTA[4], hTB[4], hTC[4];
+  // 16-byte staging (sub-tensor bits 0, 1 are the operand's positions 0, 1,
+  // at least 2^10 entries): the thread's offset of entry 4 tid
+  uint32_t lo4SA, lo4SB;
+  bool x4A, x4B;
 };
 
 __device__ __forceinline__ void ew_thread_setup(KEw d, EwThread& e) {
@@ -156,6 +165,12 @@ __device__ __forceinline__ void ew_thread_setup(KEw d, EwThread& e) {
   hi_strides(e.hTA, TBH_KF(d, tA));
   hi_strides(e.hTB, TBH_KF(d, tB));
   hi_strides(e.hTC, TBH_KF(d, tC));
+  const KI8 sA = TBH_KF(d, sA), sB = TBH_KF(d, sB);
+  constexpr bool x4 = TBH_LDS_X4 && TBH_LDS_DMA && TBH_LDS_PAD_SHIFT == 0;
+  e.x4A = x4 && d->nSA >= 10 && sA[0] == 0 && sA[1] == 1;
+  e.x4B = x4 && d->nSB >= 10 && sB[0] == 0 && sB[1] == 1;
+  e.lo4SA = lo_dep(tid, KI8{d, sA.off + 2});
+  e.lo4SB = lo_dep(tid, KI8{d, sB.off + 2});
 }
 
 // The step index and its tile range are the same in every lane of a wave;
@@ -229,13 +244,31 @@ __device__ __forceinline__ void outer_step(KEw d, uint32_t o, EwOuter& r) {
 // = lds_pad(tid) + lds_pad(256) m
 template <typename T>
 __device__ __forceinline__ void ew_stage(const TBH_GLOBAL T* __restrict__ X, int nS, uint32_t lo,
-                                         const uint32_t (&hS)[4], T* __restrict__ lX) {
+                                         const uint32_t (&hS)[4], uint32_t lo4, bool x4,
+                                         T* __restrict__ lX) {
   const int tid = threadIdx.x;
   const int n = 1 << nS;
   // recomputed per call: addresses are not held across the caller's loop
-  asm volatile("" : "+v"(lo));
+  asm volatile("" : "+v"(lo), "+v"(lo4));
 #if TBH_LDS_DMA
   static_assert(sizeof(T) == 4, "dword LDS DMA");
+  if (x4) {
+    // entries 4 l .. 4 l + 3 of lane l are 16 contiguous bytes of the
+    // operand (sub-tensor bits 0, 1 are its positions 0, 1; every other bit
+    // sits at a position >= 2, so the address is 16-byte aligned): one
+    // dwordx4 DMA per 256 entries, written at M0 + 16 x lane
+    TBH_LDS T* w = (TBH_LDS T*)lX + 4 * (tid & ~63);
+    const int cnt = n >> 10;  // 1..4, wave-uniform
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      if (m >= cnt) break;
+      const uint32_t hi = ((m & 1) ? hS[2] : 0u) + ((m & 2) ? hS[3] : 0u);
+#if defined(__HIP_DEVICE_COMPILE__)  // the host pass does not know gfx950's 16-byte form
+      __builtin_amdgcn_global_load_lds(X + lo4 + hi, w + 1024 * m, 16, 0, 0);
+#endif
+    }
+    return;
+  }
   // lane l of the wave writes entry (its wave's first tid) + l + 256 m: the
   // LDS address is wave-uniform (M0) plus 4 x lane, and a run of 64 entries
   // starting at a multiple of 64 stays contiguous under the padding
@@ -327,8 +360,8 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
         cB -= sb;
       }
     }
-    ew_stage<T>(A + cA, d->nSA, e.loSA, e.hSA, lA);
-    if (HASB) ew_stage<T>(B + cB, d->nSB, e.loSB, e.hSB, lB);
+    ew_stage<T>(A + cA, d->nSA, e.loSA, e.hSA, e.lo4SA, e.x4A, lA);
+    if (HASB) ew_stage<T>(B + cB, d->nSB, e.loSB, e.hSB, e.lo4SB, e.x4B, lB);
 #if TBH_LDS_DMA
     // the DMA writes land in LDS when the loads retire (vmcnt), which the
     // barrier alone does not wait for

@@ -181,7 +181,8 @@ struct Builder {
       };
       const bool okA = LA.size() <= 13 || aligned(LA);
       const bool okB = b < 0 || LB.size() <= 13 || aligned(LB);
-      if (na + RA <= kSubBits && nb + RB <= kSubBits && okA && okB) {
+      if (na + RA <= kSubBits && nb + RB <= kSubBits && okA && okB &&
+          (int)R.size() <= opt.direct_max_reduce) {
         T.assign(LC.begin(), LC.begin() + nTd);
         op.direct = true;
       }

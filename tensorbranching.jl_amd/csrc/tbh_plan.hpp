@@ -89,6 +89,9 @@ struct PlanOptions {
   // gemm_scatter_bits positions are row or column bits of one tile, so each
   // tile writes whole runs of 2^gemm_scatter_bits elements (< 0: never)
   int gemm_scatter_bits = 5;
+  // direct (wave-tile) EW steps reduce at most this many labels (-1: none
+  // are direct); steps with reductions run faster LDS-tiled (DMA staging)
+  int direct_max_reduce = 0;
 };
 
 // Returns 0 or sets *err and returns -1.

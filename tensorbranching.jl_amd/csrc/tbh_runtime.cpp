@@ -232,6 +232,7 @@ tbh_tuning default_tuning() {
     d.ew_chunk = 2;
     d.direct_chunk = 4;
     d.ew_per_cu = 4;
+    d.direct_max_reduce = o.direct_max_reduce;
     d.first_batch = 0.15;
     if (const char* e = std::getenv("TBH_CHAIN"))
       std::sscanf(e, "%d,%d,%d", &d.chain_big, &d.chain_side, &d.chain_reduce);
@@ -239,6 +240,8 @@ tbh_tuning default_tuning() {
       std::sscanf(e, "%d,%d", &d.split_tiles_bits, &d.split_min_k_bits);
     if (const char* e = std::getenv("TBH_EW_CHUNK")) d.ew_chunk = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("TBH_DIRECT_CHUNK")) d.direct_chunk = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("TBH_DIRECT_MAX_R"))
+      d.direct_max_reduce = std::min(kMaxReduce, std::max(-1, std::atoi(e)));
     if (const char* e = std::getenv("TBH_EW_PER_CU"))
       d.ew_per_cu = std::min(8, std::max(1, std::atoi(e)));
     if (const char* e = std::getenv("TBH_FIRST_BATCH"))
@@ -262,6 +265,7 @@ PlanOptions plan_options(const tbh_ctx* ctx, bool batched) {
     return e ? std::atoi(e) : PlanOptions().gemm_scatter_bits;
   }();
   o.gemm_scatter_bits = scatter_bits;
+  o.direct_max_reduce = t.direct_max_reduce;
   return o;
 }
 
@@ -1273,6 +1277,7 @@ static int check_tuning(const tbh_tuning& t) {
       t.chain_reduce > kMaxReduce || t.split_tiles_bits < 0 || t.split_tiles_bits > 24 ||
       t.split_min_k_bits < 4 || t.split_min_k_bits > 24 || t.ew_chunk < 1 || t.ew_chunk > 64 ||
       t.direct_chunk < 1 || t.direct_chunk > 64 || t.ew_per_cu < 1 || t.ew_per_cu > 8 ||
+      t.direct_max_reduce < -1 || t.direct_max_reduce > kMaxReduce ||
       !(t.first_batch > 0.0 && t.first_batch <= 1.0)) {
     set_error("tuning parameter out of range");
     return TBH_EINVAL;
@@ -1369,6 +1374,7 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
       {{1, 2, 4, 8}, nullptr, &tbh_tuning::ew_chunk},
       {{1, 2, 4, 8}, nullptr, &tbh_tuning::direct_chunk},
       {{2, 3, 4, 5}, nullptr, &tbh_tuning::ew_per_cu},
+      {{-1, 0, 1, 2, 3}, nullptr, &tbh_tuning::direct_max_reduce},
       {{0.05, 0.1, 0.15, 0.25, 0.4}, &tbh_tuning::first_batch, nullptr}};
   auto get = [](const tbh_tuning& t, const Param& p) { return p.d ? t.*p.d : (double)(t.*p.i); };
   auto set = [](tbh_tuning& t, const Param& p, double v) {

@@ -132,6 +132,15 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
     ref = orc.contract_slices(br, dtype)
     for mode in ("batched", "plain"):
         assert np.array_equal(backend.contract_slices(br, dtype, mode), ref), mode
+    # direct wave tiles with reductions (every eligible step direct), and no
+    # direct steps at all
+    t0 = backend.tuning()
+    try:
+        for cap in (12, -1):
+            backend.set_tuning(direct_max_reduce=cap)
+            assert np.array_equal(backend.contract_slices(br, dtype, "batched"), ref), cap
+    finally:
+        backend.set_tuning(**t0)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.int32])

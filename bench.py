@@ -13,7 +13,9 @@ like a warmup); the choice is printed.
 Printed on one JSON line: solves/sec (value), the plain per-slice GPU loop on
 the same slices, the dominant kernel's algorithmic throughput against its
 roofline (HIP events on the backend's stream), a standalone max-plus GEMM
-against the VALU ceiling, and the CPU restatement timed on a bounded sample.
+against the VALU ceiling, the serving legs, the other benchmark
+configurations (batched against plain), and the CPU restatement timed on a
+bounded sample.
 """
 import argparse
 import json
@@ -183,6 +185,51 @@ def serving_leg(tb, be, br, mis, copies, mix_count, seed):
     return out
 
 
+def configs_leg(tb, be):
+    """The other benchmark configurations, each solved BATCHED (best of 3
+    wall times, results checked equal to the plain per-slice loop's):
+    3-regular n=100 seed 0 sliced to sc 10; the unsliced 3-regular n=200
+    seed 1 network (one contraction, sc 26); the 40 x 40 King's subgraph
+    (filling 0.8, seed 3) with Int32 weights in [1, 100] sliced to sc 25,
+    also checked equal to Float32 on the same weights."""
+    def run(br, dt):
+        be.contract_slices(br, dt, "batched")
+        best, r = 1e30, None
+        for _ in range(3):
+            t = time.perf_counter()
+            r = be.contract_slices(br, dt, "batched")
+            best = min(best, time.perf_counter() - t)
+        t = time.perf_counter()
+        rp = be.contract_slices(br, dt, "plain")
+        t_plain = time.perf_counter() - t
+        if not np.array_equal(r, rp):
+            raise RuntimeError("batched and plain results differ")
+        ops = sum(2.0 * 2.0 ** tb.complexity(b).tc for b in br if b.code is not None)
+        return r, {"slices": len(br), "batched_ms": round(1e3 * best, 2),
+                   "plain_ms": round(1e3 * t_plain, 1),
+                   "batched_speedup": round(t_plain / best, 2),
+                   "tropical_GOPs": round(ops / best / 1e9, 1)}
+    out = {}
+    g = tb.random_regular_graph(100, 3, seed=0)
+    br = tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=10, order_trials=20), seed=0)
+    r, out["3regular_n100_sc10_f32"] = run(br, np.float32)
+    out["3regular_n100_sc10_f32"]["mis"] = float(r.max())
+    g = tb.random_regular_graph(200, 3, seed=1)
+    b = tb.SlicedBranch(tb.MISProblem(g), tb.initialize_code(g, order_trials=100, seed=1), 0)
+    r, out["3regular_n200_unsliced_f32"] = run([b], np.float32)
+    out["3regular_n200_unsliced_f32"].update(mis=float(r.max()), sc=tb.complexity(b).sc,
+                                              tc=round(tb.complexity(b).tc, 2))
+    g = tb.random_ksg(40, 40, 0.8, seed=3)
+    w = np.random.default_rng(3).integers(1, 101, g.nv).astype(np.int32)
+    br = tb.slice_graph(g, w, tb.ContractionTreeSlicer(sc_target=25, order_trials=50), seed=3)
+    r, out["ksg40_fill0.8_sc25_i32"] = run(br, np.int32)
+    r32 = be.contract_slices(br, np.float32, "batched")
+    if not np.array_equal(r32.astype(np.int64), r.astype(np.int64)):
+        raise RuntimeError("Int32 and Float32 results differ on integer weights")
+    out["ksg40_fill0.8_sc25_i32"]["mis"] = int(r.max())
+    return out
+
+
 def calibrate_copy():
     """A plain copy-like kernel of known size for scaling the FETCH_SIZE /
     WRITE_SIZE counters: y = -x over 2^28 floats (reads 1 GiB, writes 1 GiB),
@@ -282,6 +329,8 @@ def main():
     ap.add_argument("--serve-copies", type=int, default=3)
     ap.add_argument("--serve-mix", type=int, default=24)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (n=100, unsliced n=200, KSG Int32)")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched steps (for profiler runs)")
     args = ap.parse_args()
@@ -400,6 +449,11 @@ def main():
         serving = serving_leg(tb, be, br, mis, args.serve_copies, args.serve_mix, seed=7)
         log(f"serving: {serving}")
 
+    configs = None
+    if not args.no_configs and not args.batched_only:
+        configs = configs_leg(tb, be)
+        log(f"configs: {configs}")
+
     cpu = None
     if not args.no_cpu and not args.batched_only:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -436,6 +490,7 @@ def main():
         "roofline": roofline,
         "kernels": per_kind,
         "gemm_4096": gemm,
+        "other_configs": configs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -189,14 +189,17 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode,
  * tiles per claim of the tiled / direct EW kernels; ew_per_cu: tiled EW
  * workgroups per CU in a launch's grid (at most what is resident);
  * direct_max_reduce: EW steps reducing more labels than this run LDS-tiled,
- * not as direct wave tiles (-1: all tiled); first_batch: fraction of the
- * arena region the first batch may use.  Results never depend on them. */
+ * not as direct wave tiles (-1: all tiled); concurrent: 1 runs a level's
+ * GEMM launches on a second stream beside its EW launches; first_batch:
+ * fraction of the arena region the first batch may use.  Results never
+ * depend on them. */
 typedef struct {
   int32_t chain_big, chain_side, chain_reduce;
   int32_t split_tiles_bits, split_min_k_bits;
   int32_t ew_chunk, direct_chunk;
   int32_t ew_per_cu;
   int32_t direct_max_reduce;
+  int32_t concurrent;
   double first_batch;
 } tbh_tuning;
 int tbh_get_tuning(tbh_ctx* ctx, tbh_tuning* out);

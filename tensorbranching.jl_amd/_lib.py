@@ -44,7 +44,8 @@ class TbhTuning(C.Structure):
     _fields_ = [("chain_big", C.c_int32), ("chain_side", C.c_int32), ("chain_reduce", C.c_int32),
                 ("split_tiles_bits", C.c_int32), ("split_min_k_bits", C.c_int32),
                 ("ew_chunk", C.c_int32), ("direct_chunk", C.c_int32), ("ew_per_cu", C.c_int32),
-                ("direct_max_reduce", C.c_int32), ("first_batch", C.c_double)]
+                ("direct_max_reduce", C.c_int32), ("concurrent", C.c_int32),
+                ("first_batch", C.c_double)]
 
 
 class TbhStepInfo(C.Structure):

@@ -233,6 +233,7 @@ tbh_tuning default_tuning() {
     d.direct_chunk = 4;
     d.ew_per_cu = 4;
     d.direct_max_reduce = o.direct_max_reduce;
+    d.concurrent = 1;
     d.first_batch = 0.15;
     if (const char* e = std::getenv("TBH_CHAIN"))
       std::sscanf(e, "%d,%d,%d", &d.chain_big, &d.chain_side, &d.chain_reduce);
@@ -242,6 +243,7 @@ tbh_tuning default_tuning() {
     if (const char* e = std::getenv("TBH_DIRECT_CHUNK")) d.direct_chunk = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("TBH_DIRECT_MAX_R"))
       d.direct_max_reduce = std::min(kMaxReduce, std::max(-1, std::atoi(e)));
+    if (const char* e = std::getenv("TBH_CONCURRENT")) d.concurrent = std::atoi(e) != 0;
     if (const char* e = std::getenv("TBH_EW_PER_CU"))
       d.ew_per_cu = std::min(8, std::max(1, std::atoi(e)));
     if (const char* e = std::getenv("TBH_FIRST_BATCH"))
@@ -417,7 +419,8 @@ int ew_grid(tbh_ctx* ctx, int dtype, int64_t total, int chunk) {
 }
 
 int gemm_grid(tbh_ctx* ctx, int64_t total) {
-  return (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)ctx->num_cus * 2));
+  static const int per_cu = std::getenv("TBH_GEMM_PER_CU") ? std::atoi(std::getenv("TBH_GEMM_PER_CU")) : 2;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)ctx->num_cus * per_cu));
 }
 
 // direct steps: wave tiles, 4 waves per workgroup
@@ -686,19 +689,57 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     hipEvent_t ev = nullptr;
     // one launch per kernel kind and level (direct EW, tiled EW, GEMM); in
     // the per-step profiling mode every step alone, tagged with its shape
+    // GEMM launches on a second stream beside a level's EW launches (not
+    // while profiling): a level's steps are independent, so the VALU-bound
+    // GEMM tiles and the memory-bound EW tiles can share the CUs.  Each
+    // stream waits for the other's work of earlier levels (events recorded
+    // lazily, only when the other stream launched since the last sync).
+    const bool conc = ctx->tune.concurrent != 0 && !prof;
+    hipStream_t sk[2] = {ctx->stream, conc ? ctx->stream2 : ctx->stream};
+    auto& evpool = ctx->lvl_events[bidx & 1];
+    size_t evn = 0;
+    // dirty[s]: stream s launched work the other stream has not waited for
+    // (the upload and the leaves are stream-0 work)
+    bool dirty[2] = {true, false};
+    auto sync_to = [&](int s) {  // stream s waits for the other stream's work so far
+      const int o = 1 - s;
+      if (!conc || !dirty[o]) return hipSuccess;
+      if (evn == evpool.size()) {
+        hipEvent_t x;
+        hipError_t r = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+        if (r != hipSuccess) return r;
+        evpool.push_back(x);
+      }
+      hipEvent_t x = evpool[evn++];
+      hipError_t r = hipEventRecord(x, sk[o]);
+      if (r == hipSuccess) r = hipStreamWaitEvent(sk[s], x, 0);
+      dirty[o] = false;
+      return r;
+    };
     auto launch = [&](int k, const char* descs, const int64_t* prefix, int nops, int64_t total) {
       if (k == 0)
         return launch_ew_direct(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
                                 direct_grid(ctx, total, direct_chunk(ctx, total)),
-                                direct_chunk(ctx, total), ctx->stream);
+                                direct_chunk(ctx, total), sk[0]);
       if (k == 1)
         return launch_ew(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
                          ew_grid(ctx, dtype, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
-                         ctx->stream);
+                         sk[0]);
       return launch_gemm(dtype, true, k - 2, reinterpret_cast<const GemmDesc*>(descs), prefix,
-                         nops, total, gemm_grid(ctx, total), ctx->stream);
+                         nops, total, gemm_grid(ctx, total), sk[1]);
     };
     for (int L = 1; L <= H && e == hipSuccess; L++) {
+      if (conc) {
+        // a stream launching work of this level first waits for the other
+        // stream's work of earlier levels (both syncs before any launch)
+        bool has[2] = {false, false};
+        for (int k = 0; k < kKinds; k++)
+          if (cnt[slot(k, L)]) has[k >= 2 ? 1 : 0] = true;
+        if (has[0] && e == hipSuccess) e = sync_to(0);
+        if (has[1] && e == hipSuccess) e = sync_to(1);
+        if (has[0]) dirty[0] = true;
+        if (has[1]) dirty[1] = true;
+      }
       for (int k = 0; k < kKinds && e == hipSuccess; k++) {
         const int w = slot(k, L);
         if (!cnt[w]) continue;
@@ -733,6 +774,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
       }
       ctx->stats.waves++;
     }
+    if (e == hipSuccess && conc) e = sync_to(0);  // the last GEMMs before the epilogue
     if (e == hipSuccess) {
       e = launch_epilogue(dtype, (const EpiDesc*)(dbase + o_epi), (int64_t)nb, dbase + o_res,
                           ctx->stream);
@@ -924,10 +966,16 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
     return TBH_ENOMEM;
   }
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipStreamDestroy(ctx->stream);
+    set_error("hipStreamCreateWithFlags failed");
+    return TBH_EDEVICE;
+  }
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, pool_bytes);
   if (e != hipSuccess) {
     (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream2);
     set_error(std::string("hipMalloc pool: ") + hipGetErrorString(e));
     return TBH_ENOMEM;
   }
@@ -947,8 +995,12 @@ int tbh_destroy(tbh_ctx* ctx) {
   tbh_serve_stop(ctx);
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   if (ctx->base) (void)hipFree(ctx->base);
+  for (auto& v : ctx->lvl_events)
+    for (hipEvent_t ev : v) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   delete ctx;
   return 0;
 }
@@ -1277,7 +1329,8 @@ static int check_tuning(const tbh_tuning& t) {
       t.chain_reduce > kMaxReduce || t.split_tiles_bits < 0 || t.split_tiles_bits > 24 ||
       t.split_min_k_bits < 4 || t.split_min_k_bits > 24 || t.ew_chunk < 1 || t.ew_chunk > 64 ||
       t.direct_chunk < 1 || t.direct_chunk > 64 || t.ew_per_cu < 1 || t.ew_per_cu > 8 ||
-      t.direct_max_reduce < -1 || t.direct_max_reduce > kMaxReduce ||
+      t.direct_max_reduce < -1 || t.direct_max_reduce > kMaxReduce || t.concurrent < 0 ||
+      t.concurrent > 1 ||
       !(t.first_batch > 0.0 && t.first_batch <= 1.0)) {
     set_error("tuning parameter out of range");
     return TBH_EINVAL;
@@ -1375,6 +1428,7 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
       {{1, 2, 4, 8}, nullptr, &tbh_tuning::direct_chunk},
       {{2, 3, 4, 5}, nullptr, &tbh_tuning::ew_per_cu},
       {{-1, 0, 1, 2, 3}, nullptr, &tbh_tuning::direct_max_reduce},
+      {{0, 1}, nullptr, &tbh_tuning::concurrent},
       {{0.05, 0.1, 0.15, 0.25, 0.4}, &tbh_tuning::first_batch, nullptr}};
   auto get = [](const tbh_tuning& t, const Param& p) { return p.d ? t.*p.d : (double)(t.*p.i); };
   auto set = [](tbh_tuning& t, const Param& p, double v) {

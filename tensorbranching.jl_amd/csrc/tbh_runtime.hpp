@@ -90,6 +90,8 @@ struct tbh_ctx {
   int num_cus = 256;
   int ew_occ[2] = {4, 4};  // resident tiled-EW workgroups per CU, per element type
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // GEMM launches beside the EW launches of a level
+  std::vector<hipEvent_t> lvl_events[2];  // cross-stream level barriers, per batch parity
   char* base = nullptr;  // device pool
   tbh::Pool pool;
   tbh::Staging staging, staging2;  // descriptor uploads (ping-pong across batches)

@@ -132,13 +132,14 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
     ref = orc.contract_slices(br, dtype)
     for mode in ("batched", "plain"):
         assert np.array_equal(backend.contract_slices(br, dtype, mode), ref), mode
-    # direct wave tiles with reductions (every eligible step direct), and no
-    # direct steps at all
+    # direct wave tiles with reductions (every eligible step direct), no
+    # direct steps at all, and every launch on one stream (GEMMs not beside
+    # the EW steps of their level)
     t0 = backend.tuning()
     try:
-        for cap in (12, -1):
-            backend.set_tuning(direct_max_reduce=cap)
-            assert np.array_equal(backend.contract_slices(br, dtype, "batched"), ref), cap
+        for kw in ({"direct_max_reduce": 12}, {"direct_max_reduce": -1}, {"concurrent": 0}):
+            backend.set_tuning(**dict(t0, **kw))
+            assert np.array_equal(backend.contract_slices(br, dtype, "batched"), ref), kw
     finally:
         backend.set_tuning(**t0)
 

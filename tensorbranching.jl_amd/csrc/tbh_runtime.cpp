@@ -472,6 +472,13 @@ void parallel_for(size_t n, int threads, F f) {
 
 }  // namespace
 
+// Waits for both streams (error paths: no kernel may still use the arena
+// or the descriptors when they are released).
+static void sync_both(tbh_ctx* ctx) {
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+}
+
 // Batches run one after another on the stream through one arena region
 // (stream order keeps a batch's kernels behind the previous batch's).  A
 // batch is packed in two passes on the host while the previous batch runs:
@@ -786,14 +793,14 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     if (e == hipSuccess && done) e = hipEventRecord(done, ctx->stream);
     ctx->stats.host_ms[2] += ms_since(t_issue);
     if (e != hipSuccess) {
-      (void)hipStreamSynchronize(ctx->stream);
+      sync_both(ctx);
       ctx->pool.free(doff);
       fail(TBH_EDEVICE, std::string("batched launch: ") + hipGetErrorString(e));
       break;
     }
     // the next batch is packed while this one runs; the previous is done
     if (!finish(prev)) {
-      (void)hipStreamSynchronize(ctx->stream);
+      sync_both(ctx);
       ctx->pool.free(doff);
       break;
     }
@@ -803,7 +810,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     i0 = i1;
   }
   if (prev.on && !finish(prev) && status == 0) status = TBH_EDEVICE;
-  if (status) (void)hipStreamSynchronize(ctx->stream);
+  if (status) sync_both(ctx);
   for (auto& ev : evs)
     if (ev) (void)hipEventDestroy(ev);
   ctx->pool.free(region_off);

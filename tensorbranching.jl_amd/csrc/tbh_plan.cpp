@@ -103,12 +103,27 @@ int pos_in(const std::vector<int32_t>& v, int32_t x) {
   return -1;
 }
 
+// Sets the positions of L's labels in a label -> position table for the
+// lifetime of the object, then clears them.
+struct PosMap {
+  std::vector<int8_t>& m;
+  const std::vector<int32_t>& L;
+  PosMap(std::vector<int8_t>& m_, const std::vector<int32_t>& L_) : m(m_), L(L_) {
+    for (size_t i = 0; i < L.size(); i++) m[L[i]] = (int8_t)i;
+  }
+  ~PosMap() {
+    for (int32_t l : L) m[l] = -1;
+  }
+};
+
 struct Builder {
   const SliceIn& s;
   const PlanOptions& opt;
   SlicePlan& P;
   std::vector<std::vector<int32_t>> layout;  // per tensor
   std::string* err;
+  // label -> position in the A, B, C of the EW step being built, else -1
+  std::vector<int8_t> posA_, posB_, posC_;
 
   int add_tensor(const std::vector<int32_t>& lay) {
     PTensor t;
@@ -136,6 +151,7 @@ struct Builder {
         return fail(err, "output label missing from inputs");
     // reduced labels, ordered by their position in A then B
     std::vector<int32_t> R;
+    R.reserve(LA.size() + LB.size());
     for (int32_t l : LA)
       if (pos_in(LC, l) < 0) R.push_back(l);
     for (int32_t l : LB)
@@ -154,13 +170,17 @@ struct Builder {
       return ew(t, -1, c);
     }
     op.nR = (int)R.size();
-    auto inA = [&](int32_t l) { return pos_in(LA, l) >= 0; };
-    auto inB = [&](int32_t l) { return pos_in(LB, l) >= 0; };
+    // label -> position tables of A, B and C for this step (O(1) lookups;
+    // cleared when the step is built)
+    const PosMap mA(posA_, LA), mB(posB_, LB), mC(posC_, LC);
+    auto inA = [&](int32_t l) { return posA_[l] >= 0; };
+    auto inB = [&](int32_t l) { return posB_[l] >= 0; };
     // Ri: the reduced labels held in the sub-tensors (R's first |Ri|, i.e.
     // the lowest A positions); the rest are outer reduced labels
     std::vector<int32_t> Ri = R;
     auto inRi = [&](int32_t l) { return std::find(Ri.begin(), Ri.end(), l) != Ri.end(); };
     std::vector<int32_t> T;
+    T.reserve(kTileBits + 1);
     // direct step: the tile is C's lowest min(nP, 10) positions and the
     // operands are read straight from memory; taken when every reduced label
     // fits the sub-tensor budget and each operand is small (cache resident)
@@ -174,7 +194,7 @@ struct Builder {
       for (int p = 0; p < nTd; p++) na += inA(LC[p]), nb += inB(LC[p]);
       auto aligned = [&](const std::vector<int32_t>& L) {
         for (int p = 0; p < std::min<int>(5, (int)L.size()); p++) {
-          const int pc = pos_in(LC, L[p]);
+          const int pc = posC_[L[p]];
           if (!inRi(L[p]) && !(pc >= 0 && pc < std::min(6, nTd))) return false;
         }
         return true;
@@ -213,7 +233,7 @@ struct Builder {
           int got = 0;
           for (int32_t l : L) {
             if (got >= want) break;
-            if (inRi(l) || (pos_in(LC, l) >= 0 && try_add(l)))
+            if (inRi(l) || (posC_[l] >= 0 && try_add(l)))
               got++;
             else
               break;
@@ -237,8 +257,9 @@ struct Builder {
       };
       int best_r = -1;
       double best_c = 0;
+      std::vector<int32_t> Tt;
+      Tt.reserve(kTileBits + 1);
       for (int r_in = (int)R.size(); r_in >= 0; r_in--) {
-        std::vector<int32_t> Tt;
         const int bits = select(r_in, Tt);
         if (bits < 0) continue;
         const double c = cost((int)Tt.size(), r_in, bits % 16, b >= 0 ? bits / 16 : -100);
@@ -252,7 +273,7 @@ struct Builder {
       select(best_r, T);
     }
     std::sort(T.begin(), T.end(),
-              [&](int32_t x, int32_t y) { return pos_in(LC, x) < pos_in(LC, y); });
+              [&](int32_t x, int32_t y) { return posC_[x] < posC_[y]; });
     // a full binary tile: tile bits 0..4 stay C's lowest tile positions (a
     // lane's store covers whole 128-byte lines); bits 8..11 take MA A-only
     // then 4 - MA B-only labels when the tile has them (MA = 2, else 3 or
@@ -282,7 +303,7 @@ struct Builder {
         mid.insert(mid.end(), ao.begin() + ma, ao.end());
         mid.insert(mid.end(), bo.begin() + (4 - ma), bo.end());
         std::sort(mid.begin(), mid.end(),
-                  [&](int32_t x, int32_t y) { return pos_in(LC, x) < pos_in(LC, y); });
+                  [&](int32_t x, int32_t y) { return posC_[x] < posC_[y]; });
         T.resize(5);
         T.insert(T.end(), mid.begin(), mid.end());
         T.insert(T.end(), ao.begin(), ao.begin() + ma);
@@ -291,9 +312,12 @@ struct Builder {
       }
     }
     std::vector<int32_t> O;
+    O.reserve(LC.size());
     for (int32_t l : LC)
       if (std::find(T.begin(), T.end(), l) == T.end()) O.push_back(l);
     std::vector<int32_t> SA, SB;  // sub-tensor labels in operand order
+    SA.reserve(kSubBits + 1);
+    SB.reserve(kSubBits + 1);
     for (int32_t l : LA)
       if (inRi(l) || std::find(T.begin(), T.end(), l) != T.end()) SA.push_back(l);
     for (int32_t l : LB)
@@ -311,11 +335,11 @@ struct Builder {
     d.nSB = (int)SB.size();
     d.tiles = 1 << d.nO;
     for (int i = 0; i < d.nRo; i++) {
-      d.qA[i] = (int8_t)pos_in(LA, Ro[i]);
-      d.qB[i] = (int8_t)pos_in(LB, Ro[i]);
+      d.qA[i] = (int8_t)posA_[Ro[i]];
+      d.qB[i] = (int8_t)posB_[Ro[i]];
     }
     for (int k = 0; k < d.nT; k++) {
-      d.tC[k] = (int8_t)pos_in(LC, T[k]);
+      d.tC[k] = (int8_t)posC_[T[k]];
       d.tA[k] = (int8_t)pos_in(SA, T[k]);
       d.tB[k] = (int8_t)pos_in(SB, T[k]);
     }
@@ -323,12 +347,12 @@ struct Builder {
       d.rA[i] = (int8_t)pos_in(SA, Ri[i]);
       d.rB[i] = (int8_t)pos_in(SB, Ri[i]);
     }
-    for (int k = 0; k < d.nSA; k++) d.sA[k] = (int8_t)pos_in(LA, SA[k]);
-    for (int k = 0; k < d.nSB; k++) d.sB[k] = (int8_t)pos_in(LB, SB[k]);
+    for (int k = 0; k < d.nSA; k++) d.sA[k] = (int8_t)posA_[SA[k]];
+    for (int k = 0; k < d.nSB; k++) d.sB[k] = (int8_t)posB_[SB[k]];
     for (int k = 0; k < d.nO; k++) {
-      d.oC[k] = (int8_t)pos_in(LC, O[k]);
-      d.oA[k] = (int8_t)pos_in(LA, O[k]);
-      d.oB[k] = (int8_t)pos_in(LB, O[k]);
+      d.oC[k] = (int8_t)posC_[O[k]];
+      d.oA[k] = (int8_t)posA_[O[k]];
+      d.oB[k] = (int8_t)posB_[O[k]];
     }
     op.tiles = d.tiles;
     op.macs = std::ldexp(1.0, op.nP + op.nR);
@@ -784,7 +808,10 @@ int plan_slice(const SliceIn& s_in, const PlanOptions& opt_in, SlicePlan* out, s
     return v;
   };
 
-  Builder B{s, opt, P, {}, err};
+  Builder B{s, opt, P, {}, err, {}, {}, {}};
+  B.posA_.assign(std::max(1, s.nv), -1);
+  B.posB_.assign(std::max(1, s.nv), -1);
+  B.posC_.assign(std::max(1, s.nv), -1);
   // a node becomes up to three steps (permutes, split-K reductions)
   P.ops.reserve(3 * (size_t)nn + 4);
   P.tensors.reserve(3 * (size_t)(nl + nn) + 4);
@@ -816,6 +843,9 @@ int plan_slice(const SliceIn& s_in, const PlanOptions& opt_in, SlicePlan* out, s
     int32_t x = s.tree[2 * j], y = s.tree[2 * j + 1];
     const auto &LX = lab[x], &LY = lab[y], &LC = lab[nl + j];
     std::vector<int32_t> fx, fy;
+    fx.reserve(LC.size());
+    fy.reserve(LC.size());
+    c.t.reserve(LC.size());
     for (int32_t l : LC) {
       bool inx = pos_in(LX, l) >= 0, iny = pos_in(LY, l) >= 0;
       if (inx && iny)

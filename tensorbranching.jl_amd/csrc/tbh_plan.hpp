@@ -86,9 +86,11 @@ struct PlanOptions {
   int chain_big = 17, chain_side = 16, chain_reduce = 7;
   // a GEMM node writes its output straight in the layout its consumer asks
   // for (instead of a permute step after it) when the output's lowest
-  // gemm_scatter_bits positions are row or column bits of one tile, so each
-  // tile writes whole runs of 2^gemm_scatter_bits elements (< 0: never)
-  int gemm_scatter_bits = 5;
+  // gemm_scatter_bits positions (at least 2) are row or column bits of one
+  // tile, so each tile writes whole runs of 2^gemm_scatter_bits elements
+  // (< 0: never).  2 (16-byte runs) measured best on the 3-regular n=300
+  // slices: 0.328 s against 0.331 s for whole 128-byte lines (5).
+  int gemm_scatter_bits = 2;
   // direct (wave-tile) EW steps reduce at most this many labels (-1: none
   // are direct); steps with reductions run faster LDS-tiled (DMA staging)
   int direct_max_reduce = 0;

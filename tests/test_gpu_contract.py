@@ -188,8 +188,10 @@ def test_autotune_keeps_results(tb, orc):
         assert ms > 0 and best == be.tuning()
         assert best["chain_side"] < best["chain_big"]
         assert np.array_equal(be.contract_slices(br, np.float32), orc.contract_slices(br, np.float32))
-        with pytest.raises(tb.TbhError):
-            be.set_tuning(ew_chunk=0)
+        for bad in ({"ew_chunk": 0}, {"ew_per_cu": 9}, {"direct_max_reduce": 13},
+                    {"concurrent": 2}, {"first_batch": 0.0}):
+            with pytest.raises(tb.TbhError):
+                be.set_tuning(**bad)
         be.set_tuning(**t0)
         assert be.tuning() == t0
     finally:

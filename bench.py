@@ -122,8 +122,9 @@ def serving_leg(tb, be, br, mis, copies, mix_count, seed):
     workload submitted at once (served throughput); (b) a mixed request
     stream (serving_requests) first as a burst (capacity), then as Poisson
     arrivals at 70% of that capacity (latency p50 / p99 from the
-    scheduler's submit and completion stamps).  Every value is checked
-    against the synchronous batched call."""
+    scheduler's submit and completion stamps), then at 30%, 50% and 90%
+    (rate_sweep).  Every value is checked against the synchronous batched
+    call."""
     t = time.time()
     mix = tb.serving_requests(mix_count, seed=seed)
     want = [float(be.contract_max(b, np.float32, "batched")) for _, _, b in mix]
@@ -149,39 +150,48 @@ def serving_leg(tb, be, br, mis, copies, mix_count, seed):
         t0 = min(r[1] for r in res.values())
         t1 = max(r[2] for r in res.values())
         cap = len(mix) / (t1 - t0)
-        # Poisson arrivals at 70% of capacity, the stream twice over
-        rate = 0.7 * cap
-        rng = np.random.default_rng(seed + 1)
-        order = list(range(len(mix))) * 2
-        gaps = rng.exponential(1.0 / rate, len(order))
-        tick = []
-        start = time.perf_counter()
-        due = 0.0
-        for k, gap in zip(order, gaps):
-            due += gap
-            dt = start + due - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            tick.append((srv.submit(mix[k][2]), k))
-        res = _drain(srv, [t for t, _ in tick])
-        lat = []
-        for t, k in tick:
-            if float(res[t][0]) != want[k]:
-                raise RuntimeError("served Poisson-stream value differs from the batched call")
-            lat.append(res[t][2] - res[t][1])
-        t0 = min(r[1] for r in res.values())
-        t1 = max(r[2] for r in res.values())
+
+        def poisson(frac, stream_seed):
+            """Poisson arrivals at frac x capacity, the stream twice over."""
+            rate = frac * cap
+            rng = np.random.default_rng(stream_seed)
+            order = list(range(len(mix))) * 2
+            gaps = rng.exponential(1.0 / rate, len(order))
+            tick = []
+            start = time.perf_counter()
+            due = 0.0
+            for k, gap in zip(order, gaps):
+                due += gap
+                dt = start + due - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tick.append((srv.submit(mix[k][2]), k))
+            res = _drain(srv, [t for t, _ in tick])
+            lat = []
+            for t, k in tick:
+                if float(res[t][0]) != want[k]:
+                    raise RuntimeError("served Poisson-stream value differs from the batched call")
+                lat.append(res[t][2] - res[t][1])
+            t0 = min(r[1] for r in res.values())
+            t1 = max(r[2] for r in res.values())
+            return {"load": frac, "poisson_rate_req_per_s": round(rate, 3),
+                    "arrivals": len(order), "served_req_per_s": round(len(order) / (t1 - t0), 3),
+                    "latency_ms": {"p50": round(1e3 * _pct(lat, 0.5), 2),
+                                   "p99": round(1e3 * _pct(lat, 0.99), 2),
+                                   "max": round(1e3 * max(lat), 2)}}
+
+        # 70% of capacity, then the load swept
+        main = poisson(0.7, seed + 1)
         out["mixed_stream"] = {
             "requests": len(mix), "slices": sum(len(b) for _, _, b in mix),
             "burst_capacity_req_per_s": round(cap, 3),
-            "poisson_rate_req_per_s": round(rate, 3), "arrivals": len(order),
-            "served_req_per_s": round(len(order) / (t1 - t0), 3),
-            "latency_ms": {"p50": round(1e3 * _pct(lat, 0.5), 2),
-                           "p99": round(1e3 * _pct(lat, 0.99), 2),
-                           "max": round(1e3 * max(lat), 2)},
+            "poisson_rate_req_per_s": main["poisson_rate_req_per_s"],
+            "arrivals": main["arrivals"], "served_req_per_s": main["served_req_per_s"],
+            "latency_ms": main["latency_ms"],
             "mix": "serving_requests(%d, seed=%d): 3-regular even n ~ U[100,260] and ER n ~ "
                    "U[100,250] p=3/(n-1), sc_target ~ U[max(15, n//10-3), 25]" % (mix_count, seed),
         }
+        out["rate_sweep"] = [poisson(f, seed + 2 + i) for i, f in enumerate((0.3, 0.5, 0.9))]
     return out
 
 

@@ -5,11 +5,16 @@
 //
 //   K1 tbh_gemm_kernel    batched / grouped max-plus GEMM: 128x128, 256x64 or
 //                         256x32 tiles, 16-deep LDS slabs, 8x8 / 8x4 register
-//                         blocking, add + max3 pairs.
+//                         blocking, add + max3 pairs; the output column-major
+//                         or in the consumer's label order (row quads, or
+//                         staged through LDS).
 //   K2 tbh_ew_kernel      one wave-batched EW step: a binary tropical
 //                         contraction over bit-permuted 2^r tensors (labels of
 //                         size 2), reducing the contracted labels; also the
-//                         permute / reduce of a single tensor.
+//                         permute / reduce of a single tensor.  Sub-tensors
+//                         are staged HBM -> LDS by DMA.
+//      tbh_ew_direct_kernel  the same for small steps without LDS (wave tiles
+//                         reading straight from memory).
 //   K3 tbh_leaf_kernel    leaf tensors of the MIS network from weights.
 //   K5 tbh_epi_kernel     per-slice root + r.
 #include <hip/hip_runtime.h>

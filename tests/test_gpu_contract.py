@@ -144,6 +144,32 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
         backend.set_tuning(**t0)
 
 
+def test_fractional_float32_weights(tb, orc, backend):
+    """Float32 weights that are not integers (outside the integer-valued
+    contract).  Each tree node adds two values once per term and takes maxima,
+    and max commutes with monotone rounding, so every lowering that keeps the
+    slice's tree forms the oracle's sums bit for bit: PLAIN, and BATCHED with
+    chain re-association off (chain_big = 30).  Re-association regroups the
+    sums of a chain, (x + s1) + s2 against x + (s1 + s2); with positive
+    weights each configuration's sum of at most nv terms then moves by at
+    most 2 (nv - 1) 2^-24 sum(w), and so does the maximum over them."""
+    g = tb.random_regular_graph(160, 3, seed=5)
+    w = (np.random.default_rng(6).random(g.nv) * 10).astype(np.float32)
+    br = _slices(tb, g, 18, weights=w, seed=5, trials=30)
+    ref = orc.contract_slices(br, np.float32)
+    assert np.any(ref != np.round(ref))
+    assert np.array_equal(backend.contract_slices(br, np.float32, "plain"), ref)
+    t0 = backend.tuning()
+    try:
+        backend.set_tuning(**dict(t0, chain_big=30))
+        assert np.array_equal(backend.contract_slices(br, np.float32, "batched"), ref)
+    finally:
+        backend.set_tuning(**t0)
+    got = backend.contract_slices(br, np.float32, "batched")
+    tol = 2.0 * (g.nv - 1) * 2.0 ** -24 * float(w.astype(np.float64).sum())
+    assert np.max(np.abs(got.astype(np.float64) - ref)) <= tol
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.int32])
 def test_gemm_output_stores_bit_exact(tb, orc, backend, dtype):
     """GEMM nodes that write their output in the consumer's label order (row

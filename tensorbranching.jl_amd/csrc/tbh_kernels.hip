@@ -31,7 +31,7 @@
 #define TBH_DIRECT_WAVES 4
 #endif
 #ifndef TBH_GEMM_WAVES
-#define TBH_GEMM_WAVES 2
+#define TBH_GEMM_WAVES 3
 #endif
 
 namespace tbh {
@@ -1083,8 +1083,10 @@ __device__ __forceinline__ void gemm_tile(
 
 // Grouped: prefix[i] is the first tile of GEMM step i; a block walks tiles.
 // Every step of one launch has the tile shape SHAPE.
+// VEC kernels (every batched and plain launch) fit three resident workgroups
+// per CU without spills (<= 168 VGPRs); the scalar-load variants keep two.
 template <typename T, bool VEC, int SHAPE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TBH_GEMM_WAVES))) void
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VEC ? TBH_GEMM_WAVES : 2))) void
 tbh_gemm_kernel(const GemmDesc* __restrict__ ops, const int64_t* __restrict__ prefix, int nops,
                 int64_t total) {
   constexpr int BM = GemmShape<SHAPE>::BM, BN = GemmShape<SHAPE>::BN;
@@ -1250,6 +1252,36 @@ hipError_t launch_gemm(int dtype, bool vec, int shape, const GemmDesc* ops, cons
 #undef TBH_GEMM_SHAPES
 #undef TBH_GEMM_LAUNCH
   return hipGetLastError();
+}
+
+int gemm_blocks_per_cu(int dtype, bool vec, int shape) {
+  int n = 0;
+  hipError_t err = hipSuccess;
+#define TBH_GEMM_OCC(T, V, S) \
+  err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tbh_gemm_kernel<T, V, S>, 256, 0)
+#define TBH_GEMM_OCC_SHAPES(T, V) \
+  if (shape == 1)                 \
+    TBH_GEMM_OCC(T, V, 1);        \
+  else if (shape == 2)            \
+    TBH_GEMM_OCC(T, V, 2);        \
+  else                            \
+    TBH_GEMM_OCC(T, V, 0);
+  if (dtype == 0) {
+    if (vec) {
+      TBH_GEMM_OCC_SHAPES(float, true)
+    } else {
+      TBH_GEMM_OCC_SHAPES(float, false)
+    }
+  } else {
+    if (vec) {
+      TBH_GEMM_OCC_SHAPES(int32_t, true)
+    } else {
+      TBH_GEMM_OCC_SHAPES(int32_t, false)
+    }
+  }
+#undef TBH_GEMM_OCC_SHAPES
+#undef TBH_GEMM_OCC
+  return err == hipSuccess && n > 0 ? n : 2;
 }
 
 hipError_t launch_leaves(int dtype, const LeafDesc* leaves, int64_t n, hipStream_t s) {

@@ -615,9 +615,11 @@ int verify_plan(const SlicePlan& P, std::string* err) {
 // x = (Y', F) with F = (S', S): the small side tensors are combined first and
 // the big spine tensor Y' is read once instead of materialising Y.  F keeps
 // the labels of S' U S that occur in Y' or outside x, so x's own labels are
-// unchanged.  Accepted while F stays small and x reduces few labels.  The
-// result is the same: the tropical contraction is order-invariant (values are
-// exact), which the tests check against the unmodified tree.
+// unchanged.  Accepted while F stays small and x reduces few labels.  For
+// integer-valued tensors the result is the same bits (every sum is exact, so
+// the contraction is order-invariant), which the tests check against the
+// unmodified tree; non-integer Float32 sums regrouped this way may differ by
+// a rounding step.
 void reassociate(int32_t nl, int32_t nv, std::vector<int32_t>& tree,
                  const std::vector<std::vector<int32_t>>& lab0, const PlanOptions& opt) {
   const int32_t nn = nl - 1;

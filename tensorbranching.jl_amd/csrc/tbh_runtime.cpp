@@ -418,8 +418,12 @@ int ew_grid(tbh_ctx* ctx, int dtype, int64_t total, int chunk) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)ctx->num_cus * per_cu));
 }
 
-int gemm_grid(tbh_ctx* ctx, int64_t total) {
-  static const int per_cu = std::getenv("TBH_GEMM_PER_CU") ? std::atoi(std::getenv("TBH_GEMM_PER_CU")) : 2;
+// a persistent grid of as many workgroups as stay resident: a larger grid
+// leaves workgroups that start only when others have finished their tiles
+int gemm_grid(tbh_ctx* ctx, int64_t total, int dtype, bool vec, int shape) {
+  static const int forced =
+      std::getenv("TBH_GEMM_PER_CU") ? std::atoi(std::getenv("TBH_GEMM_PER_CU")) : 0;
+  const int per_cu = forced > 0 ? forced : ctx->gemm_occ[dtype ? 1 : 0][vec ? 1 : 0][shape];
   return (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)ctx->num_cus * per_cu));
 }
 
@@ -733,7 +737,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
                          ew_grid(ctx, dtype, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
                          sk[0]);
       return launch_gemm(dtype, true, k - 2, reinterpret_cast<const GemmDesc*>(descs), prefix,
-                         nops, total, gemm_grid(ctx, total), sk[1]);
+                         nops, total, gemm_grid(ctx, total, dtype, true, k - 2), sk[1]);
     };
     for (int L = 1; L <= H && e == hipSuccess; L++) {
       if (conc) {
@@ -906,7 +910,8 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
       ctx->stats.ew_ops_algo_bytes += op.bytes;
     } else {
       err = launch_gemm(dtype, true, gemm_shape(op.M, op.N), (const GemmDesc*)(dbase + o_op[k]),
-                        (const int64_t*)(dbase + o_pre[k]), 1, tiles[k], gemm_grid(ctx, tiles[k]),
+                        (const int64_t*)(dbase + o_pre[k]), 1, tiles[k],
+                        gemm_grid(ctx, tiles[k], dtype, true, gemm_shape(op.M, op.N)),
                         ctx->stream);
       ctx->stats.gemm_ops++;
       ctx->stats.gemm_ops_algo_ops += 2 * op.macs;
@@ -963,7 +968,11 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));
   ctx->num_cus = prop.multiProcessorCount;
-  for (int t = 0; t < 2; t++) ctx->ew_occ[t] = ew_blocks_per_cu(t);
+  for (int t = 0; t < 2; t++) {
+    ctx->ew_occ[t] = ew_blocks_per_cu(t);
+    for (int v = 0; v < 2; v++)
+      for (int s = 0; s < 3; s++) ctx->gemm_occ[t][v][s] = gemm_blocks_per_cu(t, v != 0, s);
+  }
   size_t fr = 0, tot = 0;
   HIP_TRY(hipMemGetInfo(&fr, &tot));
   if (pool_bytes == 0) pool_bytes = (size_t)(fr * 0.8);
@@ -1189,7 +1198,7 @@ int tbh_maxplus_gemm(tbh_ctx* ctx, tbh_dtype dtype, int64_t M, int64_t N, int64_
   hipEvent_t ev = ctx->profiling ? ctx->prof.begin(ctx->stream) : nullptr;
   if (e == hipSuccess)
     e = launch_gemm(dtype, vec, d.shape, (const GemmDesc*)dd, (const int64_t*)(dd + sizeof d), 1,
-                    total, gemm_grid(ctx, total), ctx->stream);
+                    total, gemm_grid(ctx, total, dtype, vec, d.shape), ctx->stream);
   if (ev)
     ctx->prof.end(ctx->stream, ev, 2, 0, 1, 4.0 * (double)batch * (M * K + K * N + M * N),
                   (double)M * N * K * batch);

@@ -34,6 +34,8 @@ hipError_t launch_epilogue(int dtype, const EpiDesc* e, int64_t n, void* out, hi
 hipError_t launch_valu_probe(int dtype, void* out, int blocks, int iters, hipStream_t s);
 // resident tiled-EW workgroups per CU (LDS and registers), from the runtime
 int ew_blocks_per_cu(int dtype);
+// resident workgroups per CU of one GEMM kernel (dtype, VEC, tile shape)
+int gemm_blocks_per_cu(int dtype, bool vec, int shape);
 
 // First-fit allocator over one device allocation (offsets, 256-byte units).
 class Pool {
@@ -89,6 +91,7 @@ struct tbh_ctx {
   int device = 0;
   int num_cus = 256;
   int ew_occ[2] = {4, 4};  // resident tiled-EW workgroups per CU, per element type
+  int gemm_occ[2][2][3] = {};  // resident GEMM workgroups per CU [dtype][vec][shape]
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // GEMM launches beside the EW launches of a level
   std::vector<hipEvent_t> lvl_events[2];  // cross-stream level barriers, per batch parity

@@ -205,9 +205,11 @@ typedef struct {
 int tbh_get_tuning(tbh_ctx* ctx, tbh_tuning* out);
 int tbh_set_tuning(tbh_ctx* ctx, const tbh_tuning* t);
 /* Autotuner: times BATCHED contractions of the given slices under candidate
- * tunings (coordinate descent from the current one, at most max_trials
- * contractions, each candidate's results checked equal to the first's),
- * installs the fastest on the context and returns it with its wall time. */
+ * tunings (coordinate descent from the current one over at most
+ * max_trials - 2 candidates, each timed as the best of two contractions
+ * whose results must equal the first's; a candidate replaces the current
+ * tuning only when 2% faster), installs the fastest on the context and
+ * returns it with its wall time. */
 int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t n,
                  int max_trials, tbh_tuning* best, double* best_ms);
 

@@ -1418,11 +1418,21 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->tune = t;
   };
+  // a tuning's time is the best of two runs: single runs vary by a few
+  // percent, enough to pass the 2% margin below by chance
+  auto time2 = [&](const tbh_tuning& t, double* ms) {
+    int r = run(t, ms);
+    if (r || res != ref) return r;  // the caller reports differing results
+    double m2 = 0;
+    r = run(t, &m2);
+    if (!r) *ms = std::min(*ms, m2);
+    return r;
+  };
   double cur_ms = 0;
   int rc = run(cur, &cur_ms);  // warm-up; its results are the reference
   if (!rc) {
     ref = res;
-    rc = run(cur, &cur_ms);
+    rc = time2(cur, &cur_ms);
   }
   if (rc) {
     restore(start);
@@ -1468,7 +1478,7 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
         set(t, p, p.values[k]);
         if (t.chain_side >= t.chain_big || check_tuning(t)) continue;
         double ms = 0;
-        rc = run(t, &ms);
+        rc = time2(t, &ms);
         trials++;
         if (rc) {
           restore(start);

@@ -423,7 +423,8 @@ int ew_grid(tbh_ctx* ctx, int dtype, int64_t total, int chunk) {
 int gemm_grid(tbh_ctx* ctx, int64_t total, int dtype, bool vec, int shape) {
   static const int forced =
       std::getenv("TBH_GEMM_PER_CU") ? std::atoi(std::getenv("TBH_GEMM_PER_CU")) : 0;
-  const int per_cu = forced > 0 ? forced : ctx->gemm_occ[dtype ? 1 : 0][vec ? 1 : 0][shape];
+  int per_cu = forced > 0 ? forced : ctx->gemm_occ[dtype ? 1 : 0][vec ? 1 : 0][shape];
+  if (per_cu <= 0) per_cu = 2;  // occupancy not queried (tbh_create fills it)
   return (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)ctx->num_cus * per_cu));
 }
 

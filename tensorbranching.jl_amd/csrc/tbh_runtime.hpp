@@ -91,7 +91,8 @@ struct tbh_ctx {
   int device = 0;
   int num_cus = 256;
   int ew_occ[2] = {4, 4};  // resident tiled-EW workgroups per CU, per element type
-  int gemm_occ[2][2][3] = {};  // resident GEMM workgroups per CU [dtype][vec][shape]
+  int gemm_occ[2][2][3] = {};  // resident GEMM workgroups per CU [dtype][vec][shape],
+                               // from the occupancy query in tbh_create
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // GEMM launches beside the EW launches of a level
   std::vector<hipEvent_t> lvl_events[2];  // cross-stream level barriers, per batch parity

@@ -22,10 +22,15 @@
  *   - Tensors are column-major: label position i has stride 2^i.
  *   - Tropical Float32: (+) = max, (x) = +, zero = -inf.  Tropical Int32:
  *     zero = TBH_I32_NEG_INF (-2^30); every stored entry below -2^29 is
- *     rewritten to it, so finite values must satisfy |x| < 2^28.
- *   - Results are bit-exact: values are integers (or, for f32, any values:
- *     max is exact and rounding is monotone), so any evaluation order of the
- *     same tree gives identical bits.
+ *     rewritten to it, so finite values must satisfy |x| < 2^28: an Int32
+ *     slice needs sum |w| + |r| < 2^28, which bounds every finite value of
+ *     its contraction (TBH_EINVAL otherwise).
+ *   - Results are bit-exact for integer-valued tensors (Float32 sums below
+ *     2^24): every sum is exact, so any evaluation order gives identical
+ *     bits.  For non-integer Float32 values PLAIN keeps the slice's tree
+ *     (max is exact and rounding monotone, so it matches any evaluation of
+ *     that tree bit for bit); BATCHED may re-associate chains of the tree,
+ *     which can move a result by a rounding step.
  */
 #ifndef TBH_H
 #define TBH_H

@@ -199,6 +199,21 @@ int convert_slice(const tbh_slice& s, tbh_dtype dtype, bool copy, SliceOwned* ou
   } else {
     o.in.wbits = nullptr;
   }
+  if (dtype == TBH_I32) {
+    // every finite entry of the contraction is a sum over a subset of the
+    // vertex weights (each vertex tensor is a leaf once), plus r at the end:
+    // sum |w| + |r| bounds them all
+    double tot = std::fabs(s.r);
+    if (o.in.wbits)
+      for (int32_t v = 0; v < s.nv; v++) tot += std::fabs((double)(int32_t)o.wbits[v]);
+    else
+      tot += s.nv;
+    if (tot >= (double)(1 << 28)) {
+      set_error("Int32 slices need sum |w| + |r| < 2^28 (a finite value of the contraction "
+                "could otherwise leave the range above the tropical zero)");
+      return TBH_EINVAL;
+    }
+  }
   if (o.in.nleaves > 0) {
     if (!s.ix_ptr || !s.ix || (o.in.nleaves > 1 && !s.tree)) {
       set_error("slice without ix_ptr / ix / tree");

@@ -76,6 +76,30 @@ def test_malformed_slices_are_rejected(tb):
                 ixx=np.array([0, 1, 0, 1, 1], np.int32)) == _lib.TBH_EINVAL
 
 
+def test_int32_weight_bound(tb):
+    """An Int32 slice needs sum |w| + |r| < 2^28 (every finite value of its
+    contraction is a subset sum of the weights plus r, and entries below
+    -2^29 are the tropical zero): just inside the bound plans, at it the
+    slice is refused before anything runs."""
+    from tensorbranching_jl_amd import _lib
+    g = tb.random_regular_graph(16, 3, seed=1)
+    code = tb.initialize_code(g, 10)
+
+    def plan(w, r=0):
+        b = tb.SlicedBranch(tb.MISProblem(g, np.asarray(w, np.int32)), code, r)
+        return tb.plan_steps(b, np.int32)
+
+    lim = 1 << 28
+    w = np.full(16, lim // 16 - 1, np.int64)
+    assert plan(w, r=15)                              # sum + |r| = 2^28 - 1
+    with pytest.raises(_lib.TbhError, match="sum"):
+        plan(w, r=16)                                 # = 2^28
+    with pytest.raises(_lib.TbhError, match="sum"):
+        plan(-w, r=-16)                               # negative weights count too
+    with pytest.raises(_lib.TbhError):
+        plan(np.full(16, lim, np.int64))              # one weight beyond 2^28
+
+
 def test_plan_steps_cover_every_node(tb):
     g = tb.random_regular_graph(60, 3, seed=2)
     b = tb.SlicedBranch(tb.MISProblem(g), tb.initialize_code(g, 8), 0)

@@ -503,6 +503,7 @@ def main():
         "other_configs": configs,
         "cpu_baseline": cpu,
     }
+    be.close()
     print(json.dumps(out))
 
 

@@ -76,8 +76,11 @@ typedef struct {
 } tbh_slice;
 
 /* Context: one GPU, one device memory pool of pool_bytes (0 = 80% of the
- * free memory), one stream.  Replaces the implicit CUDA.jl state used by
- * solve(..., usecuda=true) (src/dynamic_ob.jl:32). */
+ * free memory) and two streams at the default priority: the main stream,
+ * and a second one on which BATCHED runs a wave's GEMM launches beside its
+ * EW launches (event-ordered; every call returns with both idle).  Replaces
+ * the implicit CUDA.jl state used by solve(..., usecuda=true)
+ * (src/dynamic_ob.jl:32). */
 int tbh_create(int device, size_t pool_bytes, tbh_ctx** out);
 int tbh_destroy(tbh_ctx* ctx);
 const char* tbh_last_error(void);
@@ -101,8 +104,10 @@ int tbh_contract_max(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
  * that TropicalGEMM / CuTropicalGEMM supply (README.md:14-22):
  *   C[:,:,t] = A[:,:,t] (x) B[:,:,t],  C[m,n] = max_k A[m,k] + B[k,n]
  * column-major with leading dimensions and batch strides (in elements).
- * ptr_kind TBH_HOST: host buffers (copied through the pool);
- * TBH_DEVICE: device pointers, run on the context's stream, synchronous. */
+ * ptr_kind TBH_HOST: host buffers (copied through the pool; entries of C
+ * outside the M x N blocks -- rows M..ldc-1, gaps between batch entries --
+ * come back unchanged); TBH_DEVICE: device pointers, run on the context's
+ * stream, synchronous. */
 int tbh_maxplus_gemm(tbh_ctx* ctx, tbh_dtype dtype, int64_t M, int64_t N,
                      int64_t K, int64_t batch, const void* A, int64_t lda,
                      int64_t strideA, const void* B, int64_t ldb,
@@ -214,7 +219,10 @@ int tbh_set_tuning(tbh_ctx* ctx, const tbh_tuning* t);
  * max_trials - 2 candidates, each timed as the best of two contractions
  * whose results must equal the first's; a candidate replaces the current
  * tuning only when 2% faster), installs the fastest on the context and
- * returns it with its wall time. */
+ * returns it with its wall time.  For Float32 slices whose sums are not all
+ * exact (a non-integer weight or r, or sum |w| + |r| >= 2^24) the chain_*
+ * parameters are not searched: re-association may move such results by a
+ * rounding step, and every candidate must reproduce the first run's bits. */
 int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t n,
                  int max_trials, tbh_tuning* best, double* best_ms);
 

@@ -841,21 +841,22 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
 }
 
 // -------------------------------------------------------- PLAIN ----------
-int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SlicePlan& P,
-                  bool add_r, uint32_t* result) {
-  const uint32_t zero_bits = dtype == TBH_F32 ? f2b(0.0f) : 0u;
-  if (P.empty) {
-    *result = add_r ? slice.rbits : zero_bits;
-    return 0;
-  }
-  int64_t arena_off = ctx->pool.alloc(std::max<int64_t>(P.arena_bytes, 256));
-  if (arena_off < 0) {
-    set_error("slice arena does not fit the pool");
-    return TBH_ENOMEM;
-  }
-  const uint64_t b = (uint64_t)(ctx->base + arena_off);
+namespace {
+
+// A PLAIN slice's descriptors, packed for an arena at device address b.
+struct PlainPack {
   Pack pk;
+  size_t o_leaves = 0, o_epi = 0, o_res = 0;
+  std::vector<size_t> o_op, o_pre;
+  std::vector<int64_t> tiles;
+  int64_t nleaves = 0;
+};
+
+void pack_plain(const SliceOwned& slice, const SlicePlan& P, bool add_r, uint32_t zero_bits,
+                uint64_t b, PlainPack* out) {
+  PlainPack& q = *out;
   std::vector<LeafDesc> leaves;
+  leaves.reserve(P.leaves.size());
   for (const PLeaf& L : P.leaves) {
     LeafDesc d;
     d.p = b + (uint64_t)P.tensors[L.tensor].off;
@@ -863,71 +864,119 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
     d.wbits = (int32_t)L.wbits;
     leaves.push_back(d);
   }
+  q.nleaves = (int64_t)leaves.size();
   EpiDesc e;
   std::memset(&e, 0, sizeof e);
   e.root = b + (uint64_t)P.tensors[P.root].off;
   e.rbits = add_r ? slice.rbits : zero_bits;
-  const size_t o_leaves = pk.put(leaves.data(), leaves.size() * sizeof(LeafDesc));
-  const size_t o_epi = pk.put(&e, sizeof e);
-  const size_t o_res = pk.reserve(4);
-  std::vector<size_t> o_op(P.ops.size()), o_pre(P.ops.size());
-  std::vector<int64_t> tiles(P.ops.size());
+  q.o_leaves = q.pk.put(leaves.data(), leaves.size() * sizeof(LeafDesc));
+  q.o_epi = q.pk.put(&e, sizeof e);
+  q.o_res = q.pk.reserve(4);
+  q.o_op.resize(P.ops.size());
+  q.o_pre.resize(P.ops.size());
+  q.tiles.resize(P.ops.size());
   for (size_t k = 0; k < P.ops.size(); k++) {
     const POp& op = P.ops[k];
     int64_t pre[2] = {0, 0};
     if (op.kind == kEw) {
       EwDesc d = make_ew(op, P.tensors, b);
-      o_op[k] = pk.put(&d, sizeof d);
+      q.o_op[k] = q.pk.put(&d, sizeof d);
       pre[1] = d.tiles;
     } else {
       GemmDesc d = make_gemm(op, P.tensors, b);
-      o_op[k] = pk.put(&d, sizeof d);
+      q.o_op[k] = q.pk.put(&d, sizeof d);
       pre[1] = gemm_tiles(d);
     }
-    tiles[k] = pre[1];
-    o_pre[k] = pk.put(pre, sizeof pre);
+    q.tiles[k] = pre[1];
+    q.o_pre[k] = q.pk.put(pre, sizeof pre);
   }
-  const size_t bytes = pk.buf.size();
-  int64_t doff = ctx->pool.alloc(bytes);
-  if (doff < 0) {
-    ctx->pool.free(arena_off);
-    set_error("pool exhausted (descriptors)");
-    return TBH_ENOMEM;
+}
+
+size_t round256(size_t x) { return (x + 255) / 256 * 256; }
+
+}  // namespace
+
+// Device bytes the PLAIN executor needs for one slice: its arena, then its
+// descriptors.
+size_t plain_device_bytes(const SliceOwned& slice, const SlicePlan& P) {
+  if (P.empty) return 0;
+  PlainPack q;
+  pack_plain(slice, P, true, 0, 0, &q);
+  return round256((size_t)std::max<int64_t>(P.arena_bytes, 256)) + q.pk.buf.size();
+}
+
+// scratch == nullptr: the arena and the descriptors come from the context's
+// pool and the descriptors go up through its pinned staging buffer.
+// Otherwise they live in the caller's device memory at scratch (at least
+// plain_device_bytes), the descriptors go up from pageable host memory and
+// the result comes back the same way: nothing is allocated (no profiling).
+int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SlicePlan& P,
+                  bool add_r, uint32_t* result, char* scratch) {
+  const uint32_t zero_bits = dtype == TBH_F32 ? f2b(0.0f) : 0u;
+  if (P.empty) {
+    *result = add_r ? slice.rbits : zero_bits;
+    return 0;
   }
-  char* dbase = ctx->base + doff;
-  int rc = ctx->staging.ensure(bytes);
-  if (rc) {
-    ctx->pool.free(doff);
-    ctx->pool.free(arena_off);
-    return rc;
+  const bool ext = scratch != nullptr;
+  const size_t arena_bytes = round256((size_t)std::max<int64_t>(P.arena_bytes, 256));
+  int64_t arena_off = -1, doff = -1;
+  char* arena = scratch;
+  if (!ext) {
+    arena_off = ctx->pool.alloc(arena_bytes);
+    if (arena_off < 0) {
+      set_error("slice arena does not fit the pool");
+      return TBH_ENOMEM;
+    }
+    arena = ctx->base + arena_off;
   }
-  std::memcpy(ctx->staging.host, pk.buf.data(), bytes);
-  hipError_t err =
-      hipMemcpyAsync(dbase, ctx->staging.host, bytes, hipMemcpyHostToDevice, ctx->stream);
+  PlainPack q;
+  pack_plain(slice, P, add_r, zero_bits, (uint64_t)arena, &q);
+  const size_t bytes = q.pk.buf.size();
+  char* dbase = nullptr;
+  const void* src = q.pk.buf.data();
+  if (ext) {
+    dbase = scratch + arena_bytes;
+  } else {
+    doff = ctx->pool.alloc(bytes);
+    if (doff < 0) {
+      ctx->pool.free(arena_off);
+      set_error("pool exhausted (descriptors)");
+      return TBH_ENOMEM;
+    }
+    dbase = ctx->base + doff;
+    int rc = ctx->staging.ensure(bytes);
+    if (rc) {
+      ctx->pool.free(doff);
+      ctx->pool.free(arena_off);
+      return rc;
+    }
+    std::memcpy(ctx->staging.host, q.pk.buf.data(), bytes);
+    src = ctx->staging.host;
+  }
+  const bool prof = ctx->profiling && !ext;
+  hipError_t err = hipMemcpyAsync(dbase, src, bytes, hipMemcpyHostToDevice, ctx->stream);
   if (err == hipSuccess)
-    err = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), (int64_t)leaves.size(),
-                        ctx->stream);
+    err = launch_leaves(dtype, (const LeafDesc*)(dbase + q.o_leaves), q.nleaves, ctx->stream);
   ctx->stats.launches++;
   for (size_t k = 0; k < P.ops.size() && err == hipSuccess; k++) {
     const POp& op = P.ops[k];
-    hipEvent_t ev = ctx->profiling ? ctx->prof.begin(ctx->stream) : nullptr;
+    const int64_t tiles = q.tiles[k];
+    const int64_t* pre = (const int64_t*)(dbase + q.o_pre[k]);
+    hipEvent_t ev = prof ? ctx->prof.begin(ctx->stream) : nullptr;
     if (op.kind == kEw) {
       if (op.direct)
-        err = launch_ew_direct(dtype, (const EwDesc*)(dbase + o_op[k]),
-                               (const int64_t*)(dbase + o_pre[k]), 1, tiles[k],
-                               direct_grid(ctx, tiles[k], direct_chunk(ctx, tiles[k])),
-                               direct_chunk(ctx, tiles[k]), ctx->stream);
+        err = launch_ew_direct(dtype, (const EwDesc*)(dbase + q.o_op[k]), pre, 1, tiles,
+                               direct_grid(ctx, tiles, direct_chunk(ctx, tiles)),
+                               direct_chunk(ctx, tiles), ctx->stream);
       else
-        err = launch_ew(dtype, (const EwDesc*)(dbase + o_op[k]),
-                        (const int64_t*)(dbase + o_pre[k]), 1, tiles[k],
-                        ew_grid(ctx, dtype, tiles[k], ew_chunk(ctx, tiles[k])), ew_chunk(ctx, tiles[k]),
+        err = launch_ew(dtype, (const EwDesc*)(dbase + q.o_op[k]), pre, 1, tiles,
+                        ew_grid(ctx, dtype, tiles, ew_chunk(ctx, tiles)), ew_chunk(ctx, tiles),
                         ctx->stream);
       ctx->stats.ew_ops++;
       ctx->stats.ew_ops_algo_bytes += op.bytes;
     } else {
-      err = launch_gemm(dtype, true, gemm_shape(op.M, op.N), (const GemmDesc*)(dbase + o_op[k]),
-                        (const int64_t*)(dbase + o_pre[k]), 1, tiles[k],
-                        gemm_grid(ctx, tiles[k], dtype, true, gemm_shape(op.M, op.N)),
+      err = launch_gemm(dtype, true, gemm_shape(op.M, op.N), (const GemmDesc*)(dbase + q.o_op[k]),
+                        pre, 1, tiles, gemm_grid(ctx, tiles, dtype, true, gemm_shape(op.M, op.N)),
                         ctx->stream);
       ctx->stats.gemm_ops++;
       ctx->stats.gemm_ops_algo_ops += 2 * op.macs;
@@ -940,19 +989,24 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
   ctx->stats.total_algo_ops += 2 * P.node_macs;
   ctx->stats.leaf_tensors += P.leaves.size();
   if (err == hipSuccess)
-    err = launch_epilogue(dtype, (const EpiDesc*)(dbase + o_epi), 1, dbase + o_res, ctx->stream);
+    err = launch_epilogue(dtype, (const EpiDesc*)(dbase + q.o_epi), 1, dbase + q.o_res,
+                          ctx->stream);
   ctx->stats.launches++;
+  uint32_t res_local = 0;
+  void* res_dst = ext ? (void*)&res_local : ctx->staging.host;
   if (err == hipSuccess)
-    err = hipMemcpyAsync(ctx->staging.host, dbase + o_res, 4, hipMemcpyDeviceToHost, ctx->stream);
+    err = hipMemcpyAsync(res_dst, dbase + q.o_res, 4, hipMemcpyDeviceToHost, ctx->stream);
   hipError_t es = hipStreamSynchronize(ctx->stream);
-  if (ctx->profiling) ctx->prof.collect(ctx->stats);
-  ctx->pool.free(doff);
-  ctx->pool.free(arena_off);
+  if (prof) ctx->prof.collect(ctx->stats);
+  if (!ext) {
+    ctx->pool.free(doff);
+    ctx->pool.free(arena_off);
+  }
   if (err != hipSuccess || es != hipSuccess) {
     set_error(std::string("plain slice: ") + hipGetErrorString(err != hipSuccess ? err : es));
     return TBH_EDEVICE;
   }
-  std::memcpy(result, ctx->staging.host, 4);
+  std::memcpy(result, res_dst, 4);
   return 0;
 }
 
@@ -1166,6 +1220,11 @@ int tbh_maxplus_gemm(tbh_ctx* ctx, tbh_dtype dtype, int64_t M, int64_t N, int64_
     }
     if (K > 0) HIP_TRY(hipMemcpyAsync(dA, A, 4 * ea, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(dB, B, 4 * std::max<int64_t>(eb, 0), hipMemcpyHostToDevice, ctx->stream));
+    // C goes back as its whole span; when the span holds entries outside
+    // the M x N blocks (ldc > M, or gaps between batch entries) C is
+    // uploaded first, so that those entries come back unchanged
+    const bool dense_c = ldc == M && (batch == 1 || strideC == N * ldc);
+    if (!dense_c) HIP_TRY(hipMemcpyAsync(dC, C, 4 * ec, hipMemcpyHostToDevice, ctx->stream));
   } else if (ptr_kind != TBH_DEVICE) {
     set_error("unknown ptr_kind");
     return TBH_EINVAL;
@@ -1395,6 +1454,29 @@ int tbh_set_tuning(tbh_ctx* ctx, const tbh_tuning* t) {
   return 0;
 }
 
+// True when every finite value of the slices' contractions is an integer
+// below 2^24 in magnitude (Int32 always): then every sum is exact and any
+// grouping of the tree gives identical bits.
+static bool exact_sums(tbh_dtype dtype, const tbh_slice* slices, size_t n) {
+  if (dtype != TBH_F32) return true;
+  for (size_t i = 0; i < n; i++) {
+    const tbh_slice& s = slices[i];
+    if (s.r != std::floor(s.r)) return false;
+    double tot = std::fabs(s.r);
+    if (s.weights && s.nv > 0) {
+      const float* w = static_cast<const float*>(s.weights);
+      for (int32_t v = 0; v < s.nv; v++) {
+        if (w[v] != std::floor(w[v])) return false;
+        tot += std::fabs((double)w[v]);
+      }
+    } else {
+      tot += s.nv;
+    }
+    if (tot >= (double)(1 << 24)) return false;
+  }
+  return true;
+}
+
 // Coordinate descent over the tuning parameters, each over a short ordered
 // list of values: from the current value, its neighbours in the list are
 // timed (one BATCHED contraction each) and the fastest kept if it beats the
@@ -1472,6 +1554,15 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
       {{-1, 0, 1, 2, 3}, nullptr, &tbh_tuning::direct_max_reduce},
       {{0, 1}, nullptr, &tbh_tuning::concurrent},
       {{0.05, 0.1, 0.15, 0.25, 0.4}, &tbh_tuning::first_batch, nullptr}};
+  // chain re-association regroups a chain's sums: for Float32 slices whose
+  // sums are not all exact (non-integer weights or r, or sums reaching
+  // 2^24) a result may move by a rounding step, so the chain parameters are
+  // left out of the search and every candidate keeps the caller's grouping
+  const bool exact = exact_sums(dtype, slices, n);
+  auto searched = [&](const Param& p) {
+    return exact || (p.i != &tbh_tuning::chain_big && p.i != &tbh_tuning::chain_side &&
+                     p.i != &tbh_tuning::chain_reduce);
+  };
   auto get = [](const tbh_tuning& t, const Param& p) { return p.d ? t.*p.d : (double)(t.*p.i); };
   auto set = [](tbh_tuning& t, const Param& p, double v) {
     if (p.d)
@@ -1482,6 +1573,7 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
   for (bool improved = true; improved && trials < max_trials;) {
     improved = false;
     for (const Param& p : params) {
+      if (!searched(p)) continue;
       // neighbours of the current value in the list
       const double v0 = get(cur, p);
       size_t at = 0;

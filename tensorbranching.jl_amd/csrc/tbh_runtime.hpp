@@ -150,8 +150,11 @@ class PlanStream {
 using PlanOf = std::function<const SlicePlan*(size_t, std::string*)>;
 int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*>& slices,
                     const PlanOf& plan_of, bool add_r, uint32_t* results);
+// scratch: device memory for the slice's arena and descriptors (at least
+// plain_device_bytes) instead of the pool; nullptr = the pool
 int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SlicePlan& plan,
-                  bool add_r, uint32_t* result);
+                  bool add_r, uint32_t* result, char* scratch = nullptr);
+size_t plain_device_bytes(const SliceOwned& slice, const SlicePlan& plan);
 int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool batched,
              std::vector<SlicePlan>* plans);
 // the context's tuning as planner options

@@ -14,7 +14,9 @@ A directory holds, per finished slice `id`:
                       nested {"isleaf", "tensorindex" (1-based) | "args",
                       "eins": {"ixs", "iy"}}}, or the text `nothing`;
   weights_<id>.txt    `UnitWeight` and n, or the element type name and one
-                      weight per line (Julia prints Float32 as `1.5f0`);
+                      weight per line as Julia's string(w) prints it (`1.5`
+                      for a Float32, no `f0` suffix: the reader parses each
+                      line with parse(T, line), which rejects the suffix);
 and slices.csv with columns id,sc,tc,r.
 
 Labels and tensor indices are 1-based on disk (Julia) and 0-based here.
@@ -47,11 +49,22 @@ _WEIGHT_TYPES = {"Float64": np.float64, "Float32": np.float32, "Int64": np.int64
 def _parse_julia_number(text, dtype):
     t = text.strip()
     if np.issubdtype(dtype, np.floating):
-        # Float32 literals print with an f exponent (1.5f0, 2.0f-3)
+        # also accept Julia's Float32 literal syntax (1.5f0, 2.0f-3), which
+        # repr / show print, although string(w) does not
         if "f" in t and not t.lower().startswith(("inf", "-inf", "nan")):
             t = t.replace("f", "e")
         return dtype(float(t))
     return dtype(int(t))
+
+
+def _julia_string(x):
+    """A weight as Julia's string(w) prints it (src/io.jl:17): the shortest
+    decimal that reads back to the same value in the weight's own type, with
+    no type suffix, so that parse(T, line) (src/io.jl:40) accepts it."""
+    if isinstance(x, np.floating):
+        t = str(x)  # shortest round-trip digits in x's own precision
+        return t if any(c in t for c in ".eEn") else t + ".0"
+    return repr(x.item())
 
 
 # ---------------------------------------------------------------- weights --
@@ -69,7 +82,7 @@ def save_weights(filename, weights):
             raise TypeError(f"weights of type {w.dtype} have no Julia counterpart")
         f.write(names[w.dtype] + "\n")
         for x in w:
-            f.write(("%sf0" % repr(float(x))) if w.dtype == np.float32 else repr(x.item()))
+            f.write(_julia_string(x))
             f.write("\n")
 
 

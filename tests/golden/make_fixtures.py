@@ -5,10 +5,21 @@ these are produced by this package's generators and slicer and checked,
 before they are written, against solvers independent of the contraction:
 the exact branch-and-reduce MIS, brute force (small graphs) and the forest
 DP.  Run from the repository root:  python tests/golden/make_fixtures.py
+
+The full-size fixtures take the oracle minutes to hours of CPU time and are
+written only on request:
+  --c3    the 766 slices of the 3-regular n=300 (seed 2) graph sliced to
+          sc_target 25: every per-slice optimum, written with the slice
+          set's digest so that a test can regenerate the slices (the host
+          slicer is deterministic) and check that they are the same ones;
+  --c5    the per-request optima of the seeded serving mix
+          (serving_requests(..., seed=5)) whose oracle work fits a budget.
 """
+import hashlib
 import json
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -27,6 +38,50 @@ def dump_branches(br):
                         tree=b.code.tree.tolist())
         out.append(item)
     return out
+
+
+def sliceset_digest(br):
+    """SHA-256 over every slice's graph, weights, r and tree arrays: two
+    slice sets with the same digest hold the same contraction inputs."""
+    h = hashlib.sha256()
+    for b in br:
+        h.update(np.int64([b.p.g.nv, b.p.g.ne()]).tobytes())
+        h.update(np.ascontiguousarray(b.p.g.edges, np.int32).tobytes())
+        if b.p.weights is not None:
+            h.update(np.ascontiguousarray(np.asarray(b.p.weights), np.float64).tobytes())
+        h.update(np.float64([float(b.r)]).tobytes())
+        if b.code is not None:
+            for a in (b.code.ix_ptr, b.code.ix, b.code.tree):
+                h.update(np.ascontiguousarray(a, np.int32).tobytes())
+    return h.hexdigest()
+
+
+C3 = dict(n=300, seed=2, sc_target=25, order_trials=200)
+
+
+def c3_slices(tb):
+    g = tb.random_regular_graph(C3["n"], 3, seed=C3["seed"])
+    br = tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=C3["sc_target"],
+                                                          order_trials=C3["order_trials"]),
+                        seed=C3["seed"])
+    return g, br
+
+
+def write_c3(tb, oracle, nthreads=0):
+    """Every per-slice optimum of the n=300 workload, by the oracle."""
+    g, br = c3_slices(tb)
+    nthreads = nthreads or os.cpu_count()
+    t = time.time()
+    res = oracle.contract_slices(br, np.float32, nthreads=nthreads)
+    wall = time.time() - t
+    tcs = [None if b.code is None else round(tb.complexity(b).tc, 4) for b in br]
+    json.dump({"workload": "random 3-regular n=300 seed 2, sliced to sc_target 25 "
+                           "(order_trials 200), unit weights, Tropical Float32",
+               "params": C3, "slices": len(br), "digest": sliceset_digest(br),
+               "per_slice": res.tolist(), "mis": float(res.max()), "tc": tcs,
+               "oracle_wall_s": round(wall, 1), "oracle_threads": nthreads},
+              open(os.path.join(HERE, "c3_n300_sc25.json"), "w"))
+    print(f"c3: {len(br)} slices, MIS {res.max()}, oracle {wall:.0f}s on {nthreads} threads")
 
 
 def main():
@@ -88,4 +143,9 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if "--c3" in sys.argv:
+        import __graft_entry__
+        from oracle import oracle
+        write_c3(__graft_entry__.package(), oracle)
+    else:
+        main()

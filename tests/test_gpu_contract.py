@@ -222,3 +222,27 @@ def test_autotune_keeps_results(tb, orc):
         assert be.tuning() == t0
     finally:
         be.close()
+
+
+def test_autotune_fractional_float32_weights(tb, orc):
+    """Non-integer Float32 weights: chain re-association could move a result
+    by a rounding step, so the autotuner leaves the chain parameters where
+    the caller set them and every candidate reproduces the first run's bits.
+    Started with re-association off, the installed tuning still has it off
+    and the results equal the oracle's bit for bit."""
+    be = tb.HIPBackend(0, pool_bytes=2 << 30)
+    try:
+        g = tb.random_regular_graph(120, 3, seed=7)
+        w = (np.random.default_rng(7).random(g.nv) * 10).astype(np.float32)
+        br = _slices(tb, g, 14, weights=w, seed=7, trials=20)
+        ref = orc.contract_slices(br, np.float32)
+        assert np.any(ref != np.round(ref))
+        t0 = be.tuning()
+        be.set_tuning(**dict(t0, chain_big=30))
+        best, ms = be.autotune(br, np.float32, max_trials=8)
+        assert ms > 0 and best == be.tuning()
+        assert (best["chain_big"], best["chain_side"], best["chain_reduce"]) == \
+            (30, t0["chain_side"], t0["chain_reduce"])
+        assert np.array_equal(be.contract_slices(br, np.float32, "batched"), ref)
+    finally:
+        be.close()

@@ -88,3 +88,32 @@ def test_valu_peak_bounds_the_gemm(backend):
     ms = backend.stats()["kind_ms"][2]
     backend.set_profiling(False)
     assert 2 * n ** 3 / (ms * 1e-3) < 1.1 * backend.valu_peak(np.float32)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.int32])
+def test_gemm_padded_c_keeps_gaps(backend, orc, dtype):
+    """Host operands with ldc = M + 3 and a batch stride of C with a gap
+    after every entry: the M x N blocks hold A (x) B, and every entry of C's
+    span outside them (padding rows, gaps) comes back unchanged."""
+    import ctypes as C
+    from tensorbranching_jl_amd import _lib
+    M, N, K, nb = 37, 20, 29, 3
+    ldc, sC = M + 3, (M + 3) * N + 11
+    rng = np.random.default_rng(11)
+    A = rand_tropical(rng, (M, K, nb), dtype)
+    B = rand_tropical(rng, (K, N, nb), dtype)
+    sentinel = dtype(12345)
+    span = (nb - 1) * sC + (N - 1) * ldc + M
+    Cbuf = np.full(span + 5, sentinel, dtype)
+    code = 0 if dtype == np.float32 else 1
+    _lib.check(_lib.tbh().tbh_maxplus_gemm(backend.handle, code, M, N, K, nb, A.ctypes.data, M,
+                                           M * K, B.ctypes.data, K, K * N, Cbuf.ctypes.data, ldc,
+                                           sC, _lib.TBH_HOST))
+    ref = orc.maxplus_gemm(A, B)
+    inside = np.zeros(Cbuf.shape, bool)
+    for t in range(nb):
+        for n in range(N):
+            o = t * sC + n * ldc
+            assert np.array_equal(Cbuf[o:o + M], ref[:, n, t])
+            inside[o:o + M] = True
+    assert np.all(Cbuf[~inside] == sentinel)

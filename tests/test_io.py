@@ -52,6 +52,25 @@ def test_round_trip_float32_weights_bit_exact(tb, orc, tmp_path):
     assert np.array_equal(orc.contract_slices(br, np.float32), orc.contract_slices(back, np.float32))
 
 
+def test_float32_weights_written_as_julia_string(tb, tmp_path):
+    """save_weights writes what string(w) gives in Julia (src/io.jl:17): plain
+    decimals that parse(Float32, line) (src/io.jl:40) accepts, with no `f0`
+    suffix, each the shortest text that reads back to the same Float32."""
+    w = np.array([1.5, 2.0, 0.1, 1e-7, 3e7, 100.0], np.float32)
+    f = tmp_path / "weights_1.txt"
+    tb.save_weights(str(f), w)
+    lines = f.read_text().split()
+    assert lines[0] == "Float32"
+    assert lines[1:4] == ["1.5", "2.0", "0.1"]
+    for t, x in zip(lines[1:], w):
+        assert "f" not in t
+        assert np.float32(float(t)) == x
+    assert np.array_equal(tb.load_weights(str(f)), w)
+    # the reader also takes Julia's Float32 literal form (repr / show)
+    f.write_text("Float32\n1.5f0\n2.0f-3\n")
+    assert tb.load_weights(str(f)).tolist() == [np.float32(1.5), np.float32(2e-3)]
+
+
 def test_empty_graph_slice_round_trip(tb, tmp_path):
     empty = tb.SlicedBranch(tb.MISProblem(tb.SimpleGraph(0)), None, 7)
     tb.save_slices(str(tmp_path), [empty])
@@ -63,10 +82,10 @@ def test_empty_graph_slice_round_trip(tb, tmp_path):
 def test_julia_written_files(tb, orc, tmp_path):
     """Files as Graphs.savegraph, OMEinsum writejson, save_weights and CSV.jl
     write them: a path 1-2-3 with weights [1.5, 2, 1.25] (Float32 printed
-    with an f exponent), r = 0.5."""
+    by string(w), without a type suffix), r = 0.5."""
     (tmp_path / "graph_7.dot").write_text(
         "# written by Graphs.jl\n3,2,u,graph,Int64,simplegraph\n1,2\n2,3\n")
-    (tmp_path / "weights_7.txt").write_text("Float32\n1.5f0\n2.0f0\n1.25f0\n")
+    (tmp_path / "weights_7.txt").write_text("Float32\n1.5\n2.0\n1.25\n")
     code = {"label-type": "Int64", "inputs": [[1], [2], [3], [1, 2], [2, 3]], "output": [],
             "tree": {"isleaf": False, "eins": {"ixs": [[1, 2], [2, 3]], "iy": []},
                      "args": [

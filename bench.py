@@ -52,44 +52,125 @@ def build_workload(tb, n, seed, sc_target, trials):
 
 
 def cpu_baseline(br, gpu_results, seconds, threads):
-    """The oracle (C restatement of the reference CPU path) on a random
-    sample of slices, scaled to one solve by algorithmic work (2^tc)."""
+    """The oracle (the C restatement of the reference's CPU path: the same
+    OMEinsum lowering over a plain max-plus loop nest) timed on a stratified
+    sample of the workload's slices, on `threads` host threads and on one.
+
+    Slices are split into four strata by tc (quartiles of the slice count);
+    from each stratum slices are drawn at random (seed 0) and contracted --
+    on all threads, slices of tc >= 33 one at a time with the threads inside
+    each contraction, smaller ones one per thread -- until the stratum's
+    share of the time budget is spent (at least one draw per stratum).  A
+    stratum's time is its total work (sum of 2^tc) divided by the rate
+    (work per second) measured on its sample; the solve's time is the sum
+    over strata.  The one-thread figure samples the same way with a quarter
+    of the budget, skipping slices that would alone exceed the stratum's
+    share; a stratum left without a sample takes the rate of the nearest
+    sampled one (reported).  Every sampled result is checked equal to the
+    GPU's."""
     from oracle import oracle
     import __graft_entry__
     tb = __graft_entry__.package()
-    rng = np.random.default_rng(0)
     tcs = np.array([tb.complexity(b).tc if b.code is not None else -np.inf for b in br])
+    live = np.flatnonzero(np.isfinite(tcs))
     work = np.where(np.isfinite(tcs), 2.0 ** tcs, 0.0)
-    # slices small enough that a chunk stays within the time budget
-    order = [k for k in rng.permutation(len(br)) if tcs[k] <= 33]
-    order = np.array(order, dtype=np.int64)
-    sample, t_used = [], 0.0
-    # grow the sample until about `seconds` of CPU time has been spent
-    chunk = max(1, threads)
-    i = 0
-    t_total = 0.0
-    while i < len(order) and t_total < seconds:
-        idx = order[i:i + chunk]
-        i += chunk
-        t0 = time.time()
-        res = oracle.contract_slices([br[k] for k in idx], np.float32, nthreads=threads)
-        t_total += time.time() - t0
-        if not np.array_equal(res, gpu_results[idx]):
-            raise RuntimeError("CPU restatement and GPU differ on a sampled slice")
-        sample.extend(idx.tolist())
-    frac = work[sample].sum() / work.sum()
-    est_solve_s = t_total / frac
+    order = live[np.argsort(tcs[live], kind="stable")]
+    strata = [s for s in np.array_split(order, 4) if len(s)]
+
+    def run(nthreads, budget, guide=None):
+        rng = np.random.default_rng(0)
+        per, sampled, t_all = [], 0, 0.0
+        for si, st in enumerate(strata):
+            pick = rng.permutation(st)
+            if guide is not None:
+                # one thread: skip slices whose time alone, predicted from the
+                # stratum's rate on all threads over their count, exceeds the
+                # stratum's share of the budget (the lowest stratum keeps at
+                # least its smallest slice)
+                r1 = guide[si] / threads if guide[si] else None
+                keep = [k for k in pick if r1 and work[k] / r1 <= budget / len(strata)]
+                if not keep and si == 0:
+                    keep = [st[0]]
+                pick = keep
+            t_used, w_used, i = 0.0, 0.0, 0
+            while i < len(pick) and (i == 0 or t_used < budget / len(strata)):
+                chunk = pick[i:i + max(1, nthreads if tcs[pick[i]] < 33 else 1)]
+                i += len(chunk)
+                t0 = time.perf_counter()
+                res = oracle.contract_slices([br[k] for k in chunk], np.float32,
+                                             nthreads=nthreads)
+                t_used += time.perf_counter() - t0
+                w_used += work[chunk].sum()
+                if not np.array_equal(res, gpu_results[np.asarray(chunk)]):
+                    raise RuntimeError("CPU restatement and GPU differ on a sampled slice")
+                sampled += len(chunk)
+            per.append({"tc": [round(float(tcs[st[0]]), 2), round(float(tcs[st[-1]]), 2)],
+                        "slices": int(len(st)), "sampled": int(i),
+                        "rate": (w_used / t_used) if i else None,
+                        "work": float(work[st].sum())})
+            t_all += t_used
+        rates = [p["rate"] for p in per]
+        est = 0.0
+        for k, p in enumerate(per):
+            r = p["rate"]
+            if r is None:  # nearest sampled stratum
+                near = sorted((abs(j - k), j) for j, x in enumerate(rates) if x)
+                r = rates[near[0][1]]
+                p["rate_from_stratum"] = near[0][1]
+            est += p["work"] / r
+        return est, per, sampled, t_all
+
+    est_all, per_all, n_all, t_all = run(threads, seconds)
+    est_one, per_one, n_one, t_one = run(1, seconds / 4, guide=[p["rate"] for p in per_all])
+    fmt = lambda per: "; ".join(
+        f"tc {p['tc'][0]}-{p['tc'][1]}: {p['sampled']}/{p['slices']}"
+        + (f" (rate of stratum {p['rate_from_stratum']})" if "rate_from_stratum" in p else "")
+        for p in per)
     return {
-        "value": 1.0 / est_solve_s,
+        "value": 1.0 / est_all,
         "unit": "MIS solves/sec",
         "cores": threads,
+        "host_cpu_count": os.cpu_count(),
         "kind": "port",
-        "sample": (f"{len(sample)} of {len(br)} slices drawn at random (seed 0) among those "
-                   f"with tc <= 33, "
-                   f"{t_total:.1f}s on {threads} threads, scaled to the full solve by "
-                   f"algorithmic work 2^tc (sample = {100 * frac:.2f}% of it); "
-                   "results checked equal to the GPU's"),
+        "one_core_solves_per_s": 1.0 / est_one,
+        "sample": (f"stratified by tc (quartiles of the {len(live)} slices): on {threads} "
+                   f"threads {n_all} slices in {t_all:.1f}s [{fmt(per_all)}]; on 1 thread "
+                   f"{n_one} slices in {t_one:.1f}s [{fmt(per_one)}]; each stratum's time = "
+                   "its work (sum of 2^tc) / its sampled rate; results checked equal to the "
+                   "GPU's"),
     }
+
+
+def work_accounting(tb, br):
+    """Per-solve work of the workload's slices: the trees' own (MACs = sum of
+    2^tc, bytes = sum of 4 x 2^rwc, every label of size 2) against what the
+    BATCHED and PLAIN plans execute (sum over their steps of the configurations
+    each step visits and of 4 x (|A| + |B| + |C|)), from the planner's step
+    records (tbh_plan_steps)."""
+    tree_macs = tree_bytes = 0.0
+    ex = {m: {"ew_macs": 0.0, "ew_bytes": 0.0, "gemm_macs": 0.0, "gemm_bytes": 0.0}
+          for m in ("batched", "plain")}
+    for b in br:
+        if b.code is None:
+            continue
+        c = tb.complexity(b)
+        tree_macs += 2.0 ** c.tc
+        tree_bytes += 4.0 * 2.0 ** c.rwc
+        for m in ex:
+            for st in tb.plan_steps(b, np.float32, m):
+                k = "gemm" if st["kind"] == 1 else "ew"
+                ex[m][k + "_macs"] += st["macs"]
+                ex[m][k + "_bytes"] += st["bytes"]
+    out = {"tree": {"macs": tree_macs, "bytes": tree_bytes}}
+    for m, v in ex.items():
+        macs = v["ew_macs"] + v["gemm_macs"]
+        byts = v["ew_bytes"] + v["gemm_bytes"]
+        out[m] = dict({k: float("%.4g" % x) for k, x in v.items()},
+                      macs=float("%.4g" % macs), bytes=float("%.4g" % byts),
+                      macs_over_tree=round(macs / tree_macs, 3),
+                      bytes_over_tree=round(byts / tree_bytes, 3))
+    out["tree"] = {k: float("%.4g" % x) for k, x in out["tree"].items()}
+    return out
 
 
 def _pct(xs, q):
@@ -117,19 +198,75 @@ def _drain(srv, tickets):
     return out
 
 
-def serving_leg(tb, be, br, mis, copies, mix_count, seed):
-    """The continuous-batching scheduler: (a) `copies` requests of the main
-    workload submitted at once (served throughput); (b) a mixed request
-    stream (serving_requests) first as a burst (capacity), then as Poisson
-    arrivals at 70% of that capacity (latency p50 / p99 from the
-    scheduler's submit and completion stamps), then at 30%, 50% and 90%
-    (rate_sweep).  Every value is checked against the synchronous batched
-    call."""
+def _fixtures_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "make_fixtures", os.path.join(ROOT, "tests", "golden", "make_fixtures.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def serving_mix(tb):
+    """The serving configuration's request mix with the oracle's optima: the
+    seeded draw of tests/golden/c5_serving_mix.json, regenerated (digests
+    checked) so that every served value is compared with the oracle."""
+    mf = _fixtures_module()
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "c5_serving_mix.json")))
+    reqs, refused = mf.c5_requests(tb)
+    if refused != fx["refused"] or len(reqs) != len(fx["requests"]):
+        raise RuntimeError("serving mix differs from its fixture")
+    want = []
+    for q, f in zip(reqs, fx["requests"]):
+        if mf.sliceset_digest(q.branches) != f["digest"]:
+            raise RuntimeError(f"serving mix request {q.desc} differs from its fixture")
+        want.append(float(f["optimum"]))
+    p = fx["params"]
+    desc = (f"serving_requests({p['count']}, seed={p['seed']}): {fx['mix']}; draws refused: "
+            f"{refused['width']} for unsliced width > sc_target + {p['max_width_gap']}, "
+            f"{refused['slices']} for > {p['max_slices']} slices, {refused['work']} for "
+            f"> 2^{p['max_log2_macs']} MACs; oracle-checked values")
+    return reqs, want, desc
+
+
+def _served(srv, reqs, order, want, rate=None, rng=None):
+    """Submits reqs[k] for k in order (Poisson arrivals at `rate`, or all at
+    once), polls them to completion and checks each value; returns
+    (latencies, window seconds)."""
+    tick = []
+    start = time.perf_counter()
+    due = 0.0
+    for k in order:
+        if rate:
+            due += rng.exponential(1.0 / rate)
+            dt = start + due - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+        tick.append((srv.submit(reqs[k].branches, reqs[k].dtype), k))
+    res = _drain(srv, [t for t, _ in tick])
+    lat = []
+    for t, k in tick:
+        if float(res[t][0]) != want[k]:
+            raise RuntimeError(f"served value of {reqs[k].desc} differs from the oracle's")
+        lat.append(res[t][2] - res[t][1])
+    t0 = min(r[1] for r in res.values())
+    t1 = max(r[2] for r in res.values())
+    return lat, t1 - t0
+
+
+def serving_leg(tb, be, br, mis, copies, seed):
+    """The continuous-batching scheduler over the 200 GB pool: (a) `copies`
+    requests of the main workload submitted at once (served throughput);
+    (b) the serving configuration's request mix (serving_mix) first as a
+    burst (capacity), then as Poisson arrivals at 70% of that capacity
+    (latency p50 / p99 from the scheduler's submit and completion stamps),
+    then at 30%, 50% and 90% (rate_sweep), every value checked against the
+    oracle's; (c) the latency lane: a one-slice priority request submitted
+    while a main-workload request is being served."""
     t = time.time()
-    mix = tb.serving_requests(mix_count, seed=seed)
-    want = [float(be.contract_max(b, np.float32, "batched")) for _, _, b in mix]
-    log(f"serving mix: {mix_count} requests, "
-        f"{sum(len(b) for _, _, b in mix)} slices, generated in {time.time() - t:.1f}s")
+    mix, want, desc = serving_mix(tb)
+    log(f"serving mix: {len(mix)} requests, {sum(len(q.branches) for q in mix)} slices, "
+        f"generated in {time.time() - t:.1f}s")
     out = {}
     with tb.Server(be) as srv:
         # (a) main workload, burst
@@ -141,57 +278,42 @@ def serving_leg(tb, be, br, mis, copies, mix_count, seed):
             raise RuntimeError("served MIS differs from the batched call")
         out["main_burst"] = {"requests": copies, "solves_per_s": round(copies / (t1 - t0), 4),
                              "window_s": round(t1 - t0, 3)}
-        # (b) mixed stream: burst for capacity
-        tick = [srv.submit(b) for _, _, b in mix]
-        res = _drain(srv, tick)
-        for t, w in zip(tick, want):
-            if float(res[t][0]) != w:
-                raise RuntimeError("served mixed-stream value differs from the batched call")
-        t0 = min(r[1] for r in res.values())
-        t1 = max(r[2] for r in res.values())
-        cap = len(mix) / (t1 - t0)
+        # (b) the mix: burst for capacity, then Poisson arrivals
+        lat, window = _served(srv, mix, range(len(mix)), want)
+        cap = len(mix) / window
 
         def poisson(frac, stream_seed):
-            """Poisson arrivals at frac x capacity, the stream twice over."""
             rate = frac * cap
-            rng = np.random.default_rng(stream_seed)
             order = list(range(len(mix))) * 2
-            gaps = rng.exponential(1.0 / rate, len(order))
-            tick = []
-            start = time.perf_counter()
-            due = 0.0
-            for k, gap in zip(order, gaps):
-                due += gap
-                dt = start + due - time.perf_counter()
-                if dt > 0:
-                    time.sleep(dt)
-                tick.append((srv.submit(mix[k][2]), k))
-            res = _drain(srv, [t for t, _ in tick])
-            lat = []
-            for t, k in tick:
-                if float(res[t][0]) != want[k]:
-                    raise RuntimeError("served Poisson-stream value differs from the batched call")
-                lat.append(res[t][2] - res[t][1])
-            t0 = min(r[1] for r in res.values())
-            t1 = max(r[2] for r in res.values())
+            lat, window = _served(srv, mix, order, want, rate, np.random.default_rng(stream_seed))
             return {"load": frac, "poisson_rate_req_per_s": round(rate, 3),
-                    "arrivals": len(order), "served_req_per_s": round(len(order) / (t1 - t0), 3),
+                    "arrivals": len(order), "served_req_per_s": round(len(order) / window, 3),
                     "latency_ms": {"p50": round(1e3 * _pct(lat, 0.5), 2),
                                    "p99": round(1e3 * _pct(lat, 0.99), 2),
                                    "max": round(1e3 * max(lat), 2)}}
 
-        # 70% of capacity, then the load swept
         main = poisson(0.7, seed + 1)
         out["mixed_stream"] = {
-            "requests": len(mix), "slices": sum(len(b) for _, _, b in mix),
+            "requests": len(mix), "slices": sum(len(q.branches) for q in mix),
             "burst_capacity_req_per_s": round(cap, 3),
             "poisson_rate_req_per_s": main["poisson_rate_req_per_s"],
             "arrivals": main["arrivals"], "served_req_per_s": main["served_req_per_s"],
-            "latency_ms": main["latency_ms"],
-            "mix": "serving_requests(%d, seed=%d): 3-regular even n ~ U[100,260] and ER n ~ "
-                   "U[100,250] p=3/(n-1), sc_target ~ U[max(15, n//10-3), 25]" % (mix_count, seed),
-        }
+            "latency_ms": main["latency_ms"], "mix": desc}
         out["rate_sweep"] = [poisson(f, seed + 2 + i) for i, f in enumerate((0.3, 0.5, 0.9))]
+        # (c) latency lane behind a main-workload request in flight
+        one = [b for b in br if b.code is not None][:1]
+        want_one = float(be.contract_slices(one, np.float32, "batched")[0])
+        t_big = srv.submit(br)
+        time.sleep(0.03)
+        t_fast = srv.submit(one, priority=1)
+        res = _drain(srv, [t_big, t_fast])
+        if float(res[t_fast][0]) != want_one or float(res[t_big][0]) != float(mis):
+            raise RuntimeError("latency-lane values differ from the batched call")
+        out["latency_lane"] = {
+            "priority_request_ms": round(1e3 * (res[t_fast][2] - res[t_fast][1]), 2),
+            "main_request_ms": round(1e3 * (res[t_big][2] - res[t_big][1]), 2),
+            "note": "one slice (priority 1) submitted 30 ms after a main-workload request"}
+        out["scheduler"] = srv.stats()
     return out
 
 
@@ -337,7 +459,6 @@ def main():
     ap.add_argument("--autotune-trials", type=int, default=12,
                     help="contractions the autotuner may time before the timed steps (0: off)")
     ap.add_argument("--serve-copies", type=int, default=3)
-    ap.add_argument("--serve-mix", type=int, default=24)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (n=100, unsliced n=200, KSG Int32)")
@@ -456,7 +577,7 @@ def main():
 
     serving = None
     if not args.no_serve and not args.batched_only:
-        serving = serving_leg(tb, be, br, mis, args.serve_copies, args.serve_mix, seed=7)
+        serving = serving_leg(tb, be, br, mis, args.serve_copies, seed=7)
         log(f"serving: {serving}")
 
     configs = None
@@ -466,8 +587,13 @@ def main():
 
     cpu = None
     if not args.no_cpu and not args.batched_only:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the threads the host gives this process: OMP_NUM_THREADS where it
+        # is set (the GPU box allots 16 host threads per GPU), else every core
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or \
+            (os.cpu_count() or 1)
         cpu = cpu_baseline(br, ref, args.cpu_seconds, threads)
+
+    work = None if args.batched_only else work_accounting(tb, br)
 
     lat_ms = sorted(1e3 * x for x in lat)
     out = {
@@ -499,6 +625,7 @@ def main():
         "tropical_GOPs": round(st_timed["total_algo_ops"] / t_total / 1e9, 1),
         "roofline": roofline,
         "kernels": per_kind,
+        "work_per_solve": work,
         "gemm_4096": gemm,
         "other_configs": configs,
         "cpu_baseline": cpu,

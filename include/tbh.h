@@ -137,8 +137,8 @@ typedef struct {
   size_t n;
   int32_t priority;         /* 0: normal; 1: latency lane (a caller blocked on
                              * one slice, as slice_dfs_lp, src/slice.jl:39):
-                             * its slices go into the next round ahead of
-                             * every normal request's */
+                             * its slices go into the next round formed,
+                             * ahead of every normal request's */
 } tbh_request;
 
 typedef enum {
@@ -148,10 +148,30 @@ typedef enum {
   TBH_FAILED = 3
 } tbh_status;
 
-/* Start the scheduler thread; requests are admitted while the summed
- * footprint of resident slices stays within the pool and at most
- * max_resident_slices are resident. */
+/* Start the planner and scheduler threads.  A request is refused alone
+ * (FAILED) when one of its slices' arenas (tbh_slice_footprint) exceeds the
+ * region an idle pool gives a batched run.  The scheduler runs rounds: a
+ * round admits slices -- latency-lane requests first, then arrival order --
+ * while their summed arenas fit that region, their summed estimated work
+ * (the MACs of the slices' trees) stays within 2^round_log2_macs and at
+ * most max_resident_slices (0: no cap) are in it.  Rounds are pipelined on
+ * the stream (the next one is enqueued while one runs), so a new request
+ * waits for at most the round running and the one queued.  A round whose
+ * execution fails is run again request by request, and only the requests
+ * that fail alone end FAILED.  While requests are in flight the scheduler
+ * holds the context: synchronous calls on it wait until serving is idle. */
+#define TBH_SERVE_ROUND_LOG2_MACS 38
+typedef struct {
+  size_t max_resident_slices; /* slices per round, 0: no cap                */
+  double round_log2_macs;     /* round work bound (log2 MACs), 0: default   */
+} tbh_serve_config;
+int tbh_serve_start_ex(tbh_ctx* ctx, const tbh_serve_config* cfg);
+/* tbh_serve_start_ex with the default round work bound. */
 int tbh_serve_start(tbh_ctx* ctx, size_t max_resident_slices);
+/* Rounds run, rounds run again request by request after a failed
+ * execution, and requests that ended FAILED, since serving started. */
+int tbh_serve_stats(tbh_ctx* ctx, uint64_t* rounds, uint64_t* retried_rounds,
+                    uint64_t* failed_requests);
 int tbh_serve_stop(tbh_ctx* ctx);
 /* Thread-safe (many producers). */
 int tbh_submit(tbh_ctx* ctx, const tbh_request* req, uint64_t* ticket);

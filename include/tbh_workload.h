@@ -73,6 +73,13 @@ int tbw_elimination_order(const tbw_graph* g, int32_t trials, uint64_t seed,
 tbw_sliceset* tbw_slice(const tbw_graph* g, const double* weights,
                         int32_t sc_target, int32_t order_trials, uint64_t seed,
                         int32_t nthreads, int32_t kernelize);
+/* tbw_slice that gives up (NULL, "more than max_slices slices") as soon as
+ * the finished and open branches outnumber max_slices; tbw_slice uses
+ * 2^18. */
+tbw_sliceset* tbw_slice_limited(const tbw_graph* g, const double* weights,
+                                int32_t sc_target, int32_t order_trials,
+                                uint64_t seed, int32_t nthreads, int32_t kernelize,
+                                int64_t max_slices);
 /* The unsliced network of g with its tree (a single slice, r = 0). */
 tbw_sliceset* tbw_whole(const tbw_graph* g, const double* weights,
                         int32_t order_trials, uint64_t seed);

@@ -31,6 +31,13 @@
 
 static __thread char g_err[512];
 const char* orc_last_error(void) { return g_err; }
+
+/* Threads inside one contraction (GEMM columns, permutes): set by
+ * orc_contract_slices while it contracts a large slice on its own; 1 while
+ * slices run one per thread.  Only loops whose iterations write disjoint
+ * outputs are split, so results do not depend on it. */
+static int g_inner = 1;
+#define ORC_INNER_MIN ((int64_t)1 << 16) /* work below which a loop stays serial */
 #define FAIL(...)                                    \
   do {                                               \
     snprintf(g_err, sizeof g_err, __VA_ARGS__);      \
@@ -264,12 +271,16 @@ int orc_maxplus_gemm_f32(int64_t M, int64_t N, int64_t K, int64_t batch,
                          int64_t ldc, int64_t sC) {
   if (M < 0 || N < 0 || K < 0 || batch < 0) FAIL("negative size");
   if (lda < M || ldb < K || ldc < M) FAIL("leading dimension too small");
+  /* each (t, n) column of C is one iteration: disjoint outputs */
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(static) \
+    if (g_inner > 1 && batch * N * M * (K + 1) >= ORC_INNER_MIN) num_threads(g_inner)
+#endif
   for (int64_t t = 0; t < batch; t++) {
-    const float* a = A + t * sA;
-    const float* b = B + t * sB;
-    float* c = C + t * sC;
     for (int64_t n = 0; n < N; n++) {
-      float* cc = c + n * ldc;
+      const float* a = A + t * sA;
+      const float* b = B + t * sB;
+      float* cc = C + t * sC + n * ldc;
       for (int64_t m = 0; m < M; m++) cc[m] = -INFINITY;
       for (int64_t k = 0; k < K; k++) {
         const float bk = b[k + n * ldb];
@@ -294,12 +305,16 @@ int orc_maxplus_gemm_i32(int64_t M, int64_t N, int64_t K, int64_t batch,
                          int64_t ldc, int64_t sC) {
   if (M < 0 || N < 0 || K < 0 || batch < 0) FAIL("negative size");
   if (lda < M || ldb < K || ldc < M) FAIL("leading dimension too small");
+  /* each (t, n) column of C is one iteration: disjoint outputs */
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(static) \
+    if (g_inner > 1 && batch * N * M * (K + 1) >= ORC_INNER_MIN) num_threads(g_inner)
+#endif
   for (int64_t t = 0; t < batch; t++) {
-    const int32_t* a = A + t * sA;
-    const int32_t* b = B + t * sB;
-    int32_t* c = C + t * sC;
     for (int64_t n = 0; n < N; n++) {
-      int32_t* cc = c + n * ldc;
+      const int32_t* a = A + t * sA;
+      const int32_t* b = B + t * sB;
+      int32_t* cc = C + t * sC + n * ldc;
       for (int64_t m = 0; m < M; m++) cc[m] = INT32_MIN;
       for (int64_t k = 0; k < K; k++) {
         const int32_t bk = b[k + n * ldb];
@@ -351,10 +366,22 @@ static int permute_reduce(int dtype, int ns, const int32_t* sl, const void* src,
     hit[i] = j;
   }
   int64_t ndst = (int64_t)1 << nd;
+  /* distinct h reach disjoint dst entries when every hi bit is kept */
+  int hi_kept = 1;
+  for (int b = lo; b < ns; b++)
+    if (dpos[b] < 0) hi_kept = 0;
+  const int par = g_inner > 1 && hi_kept && ((int64_t)1 << ns) >= ORC_INNER_MIN;
+  (void)par;
   if (dtype == ORC_F32) {
     const float* s = (const float*)src;
     float* d = (float*)dst;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if (par) num_threads(g_inner)
+#endif
     for (int64_t j = 0; j < ndst; j++) d[j] = -INFINITY;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if (par) num_threads(g_inner)
+#endif
     for (int64_t h = 0; h < nhi; h++)
       for (int64_t l = 0; l < nlo; l++) {
         int64_t j = hit[h] | lot[l];
@@ -364,7 +391,13 @@ static int permute_reduce(int dtype, int ns, const int32_t* sl, const void* src,
   } else {
     const int32_t* s = (const int32_t*)src;
     int32_t* d = (int32_t*)dst;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if (par) num_threads(g_inner)
+#endif
     for (int64_t j = 0; j < ndst; j++) d[j] = INT32_MIN;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if (par) num_threads(g_inner)
+#endif
     for (int64_t h = 0; h < nhi; h++)
       for (int64_t l = 0; l < nlo; l++) {
         int64_t j = hit[h] | lot[l];
@@ -541,14 +574,54 @@ int orc_solve_slice(int dtype, const orc_slice* s, void* result) {
 
 /* contract_slices (src/dynamic_ob.jl:36-48): empty graph -> r, otherwise
  * solve_slice + r in the element type. */
+static int contract_one(int dtype, const orc_slice* si, void* result) {
+  if (si->nv == 0) {
+    if (dtype == ORC_F32)
+      *(float*)result = (float)si->r;
+    else
+      *(int32_t*)result = (int32_t)si->r;
+    return 0;
+  }
+  union {
+    float f;
+    int32_t i;
+  } t;
+  if (orc_solve_slice(dtype, si, &t)) return -1;
+  if (dtype == ORC_F32)
+    *(float*)result = t.f + (float)si->r;
+  else
+    *(int32_t*)result = t.i <= ORC_I32_NEG_THRESH ? ORC_I32_NEG : t.i + (int32_t)si->r;
+  return 0;
+}
+
+/* Slices of at least 2^ORC_BIG_TC configurations are contracted one at a
+ * time with the threads inside each contraction, the rest one per thread. */
+#define ORC_BIG_TC 33.0
+
 int orc_contract_slices(int dtype, const orc_slice* s, int64_t n,
                         void* results, int nthreads) {
   int err = 0;
 #ifdef _OPENMP
   if (nthreads <= 0) nthreads = omp_get_max_threads();
+#endif
+  char* big = (char*)calloc(n > 0 ? (size_t)n : 1, 1);
+  if (nthreads > 1)
+    for (int64_t i = 0; i < n; i++) {
+      double tc, sc, rw;
+      if (s[i].nv > 0 && s[i].nleaves > 1 &&
+          orc_complexity(s[i].nleaves, s[i].ix_ptr, s[i].ix, s[i].tree, &tc, &sc, &rw) == 0 &&
+          tc >= ORC_BIG_TC)
+        big[i] = 1;
+    }
+  g_inner = nthreads;
+  for (int64_t i = 0; i < n; i++)
+    if (big[i] && contract_one(dtype, &s[i], (char*)results + 4 * i)) err = 1;
+  g_inner = 1;
+#ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
 #endif
   for (int64_t i = 0; i < n; i++) {
+    if (big[i]) continue;
     if (s[i].nv == 0) {
       if (dtype == ORC_F32)
         ((float*)results)[i] = (float)s[i].r;
@@ -573,6 +646,7 @@ int orc_contract_slices(int dtype, const orc_slice* s, int64_t n,
       ((int32_t*)results)[i] =
           t.i <= ORC_I32_NEG_THRESH ? ORC_I32_NEG : t.i + (int32_t)s[i].r;
   }
+  free(big);
   if (err) FAIL("contract_slices: a slice failed");
   return 0;
 }

@@ -17,7 +17,7 @@ from .api import (  # noqa: F401
     solve_slice, tc)
 from .workload import (  # noqa: F401
     ContractionTreeSlicer, dynamic_ob_mis, erdos_renyi, initialize_code, random_ksg,
-    random_regular_graph, serving_requests, slice_graph)
+    random_regular_graph, serving_requests, slice_graph, SERVING_MIX, ServingRequest)
 from .serve import Server  # noqa: F401
 from .io import (  # noqa: F401
     load_all_finished, load_code, load_finished, load_weights, loadgraph, save_code,

@@ -25,6 +25,10 @@ class TbhSlice(C.Structure):
                 ("r", C.c_double)]
 
 
+class TbhServeConfig(C.Structure):
+    _fields_ = [("max_resident_slices", C.c_size_t), ("round_log2_macs", C.c_double)]
+
+
 class TbhRequest(C.Structure):
     _fields_ = [("dtype", C.c_int), ("slices", C.POINTER(TbhSlice)), ("n", C.c_size_t),
                 ("priority", C.c_int32)]
@@ -100,6 +104,9 @@ TBH_SIGNATURES = {
     "tbh_plan_steps": (C.c_int, [C.POINTER(TbhSlice), C.c_int, C.c_int, C.POINTER(TbhStepInfo),
                                  C.c_int]),
     "tbh_reset_stats": (C.c_int, [C.c_void_p]),
+    "tbh_serve_start_ex": (C.c_int, [C.c_void_p, C.POINTER(TbhServeConfig)]),
+    "tbh_serve_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint64)]),
     "tbh_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
@@ -116,6 +123,8 @@ TBW_SIGNATURES = {
     "tbw_elimination_order": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_void_p]),
     "tbw_slice": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint64,
                                C.c_int32, C.c_int32]),
+    "tbw_slice_limited": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint64,
+                                       C.c_int32, C.c_int32, C.c_int64]),
     "tbw_whole": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64]),
     "tbw_sliceset_count": (C.c_int64, [C.c_void_p]),
     "tbw_sliceset_get": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TbwSliceView)]),

@@ -492,11 +492,12 @@ void parallel_for(size_t n, int threads, F f) {
 
 }  // namespace
 
-// Waits for both streams (error paths: no kernel may still use the arena
-// or the descriptors when they are released).
-static void sync_both(tbh_ctx* ctx) {
-  (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+// The arena region of a batched run when `largest` bytes are the largest free
+// block of the pool: that block less a reserve for the step descriptors.
+size_t region_bytes(size_t largest) {
+  const size_t reserve = std::min<size_t>(
+      (size_t)8 << 30, std::max(largest / 8, std::min<size_t>(largest / 4, (size_t)16 << 20)));
+  return largest - reserve;
 }
 
 // Batches run one after another on the stream through one arena region
@@ -508,55 +509,347 @@ static void sync_both(tbh_ctx* ctx) {
 // descriptor straight into its final place in a pinned staging buffer (two,
 // used alternately), which goes to the device in one copy.  The previous
 // batch is waited for only after the next one is enqueued.
+BatchRunner::BatchRunner(tbh_ctx* ctx) : ctx_(ctx) {}
+
+BatchRunner::~BatchRunner() { close(); }
+
+int BatchRunner::open() {
+  if (region_off_ >= 0) return 0;
+  const size_t largest = ctx_->pool.largest_free();
+  region_ = largest >= 256 ? region_bytes(largest) : 0;
+  region_off_ = region_ >= 256 ? ctx_->pool.alloc(region_) : -1;
+  if (region_off_ < 0) {
+    region_ = 0;
+    set_error("pool too small for a slice arena");
+    return TBH_ENOMEM;
+  }
+  arena_base_ = (uint64_t)(ctx_->base + region_off_);
+  for (auto& ev : evs_)
+    if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+  return 0;
+}
+
+void BatchRunner::close() {
+  if (prev_.on) {
+    // an unfinished batch: wait for it without reporting (error paths)
+    sync_all();
+    ctx_->pool.free(prev_.doff);
+    prev_.on = false;
+  }
+  for (auto& ev : evs_)
+    if (ev) (void)hipEventDestroy(ev);
+  evs_[0] = evs_[1] = nullptr;
+  if (region_off_ >= 0) ctx_->pool.free(region_off_);
+  region_off_ = -1;
+  region_ = 0;
+  if (ctx_->profiling) ctx_->prof.collect(ctx_->stats);
+}
+
+// Waits for both streams (error paths: no kernel may still use the arena
+// or the descriptors when they are released).
+void BatchRunner::sync_all() {
+  (void)hipStreamSynchronize(ctx_->stream);
+  if (ctx_->stream2) (void)hipStreamSynchronize(ctx_->stream2);
+}
+
+// Waits for the batch in flight, releases its descriptors and copies its
+// results to their destination.
+int BatchRunner::finish(void** done_tag) {
+  if (done_tag) *done_tag = nullptr;
+  if (!prev_.on) return 0;
+  auto t_wait = std::chrono::steady_clock::now();
+  hipError_t e = prev_.done ? hipEventSynchronize(prev_.done) : hipStreamSynchronize(ctx_->stream);
+  ctx_->stats.host_ms[3] += ms_since(t_wait);
+  ctx_->pool.free(prev_.doff);
+  prev_.on = false;
+  if (done_tag) *done_tag = prev_.tag;
+  if (e != hipSuccess) {
+    set_error(std::string("batched execution: ") + hipGetErrorString(e));
+    return TBH_EDEVICE;
+  }
+  std::memcpy(prev_.dst, ctx_->results_pin[prev_.parity].host, 4 * prev_.nb);
+  return 0;
+}
+
+int BatchRunner::flush(void** done_tag) { return finish(done_tag); }
+
+int BatchRunner::enqueue(int dtype, const std::vector<const SliceOwned*>& slices,
+                         const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* dst,
+                         void* tag, void** done_tag) {
+  if (done_tag) *done_tag = nullptr;
+  tbh_ctx* ctx = ctx_;
+  const size_t nb = plans.size();
+  if (region_off_ < 0) {
+    set_error("batched run not open");
+    return TBH_EINVAL;
+  }
+  if (nb == 0) return 0;
+  const uint32_t zero_bits = dtype == TBH_F32 ? f2b(0.0f) : 0u;
+  const bool per_step = ctx->profiling && std::getenv("TBH_PROFILE_STEPS");
+  const bool prof = ctx->profiling;
+  const int bidx = bidx_++;
+  auto t_pack = std::chrono::steady_clock::now();
+  std::vector<uint64_t> sbase(nb);
+  int H = 0;
+  {
+    uint64_t acc = 0;
+    for (size_t i = 0; i < nb; i++) {
+      sbase[i] = arena_base_ + acc;
+      acc += plans[i]->empty ? 0 : (uint64_t)plans[i]->arena_bytes;
+      H = std::max(H, plans[i]->nlevels);
+    }
+    if (acc > region_) {
+      set_error("batch arenas (" + std::to_string(acc) + " bytes) exceed the region (" +
+                std::to_string(region_) + ")");
+      return TBH_ENOMEM;
+    }
+  }
+  const int W = kKinds * (H + 1);  // (kind, level) slots
+  auto slot = [H](int k, int L) { return k * (H + 1) + L; };
+  // pass 1
+  std::vector<int32_t> cnt(W, 0), start(nb * W);
+  std::vector<std::vector<int64_t>> pfx(W, std::vector<int64_t>(1, 0));
+  std::vector<double> lvl_bytes(W, 0.0), lvl_macs(W, 0.0);
+  std::vector<std::vector<std::pair<double, double>>> step_work(per_step ? W : 0);
+  std::vector<int64_t> leaf_start(nb + 1, 0);
+  for (size_t i = 0; i < nb; i++) {
+    const SlicePlan& P = *plans[i];
+    std::memcpy(&start[i * W], cnt.data(), W * sizeof(int32_t));
+    leaf_start[i + 1] = leaf_start[i] + (P.empty ? 0 : (int64_t)P.leaves.size());
+    if (P.empty) continue;
+    for (const POp& op : P.ops) {
+      const int w = slot(op_kind(op), op.level);
+      cnt[w]++;
+      pfx[w].push_back(pfx[w].back() + op_tiles(op));
+      lvl_bytes[w] += op.bytes;
+      lvl_macs[w] += op.macs;
+      if (per_step) step_work[w].push_back({op.bytes, op.macs});
+      if (op.kind == kEw) {
+        ctx->stats.ew_ops++;
+        ctx->stats.ew_ops_algo_bytes += op.bytes;
+      } else {
+        ctx->stats.gemm_ops++;
+        ctx->stats.gemm_ops_algo_ops += 2 * op.macs;
+      }
+    }
+    ctx->stats.total_algo_ops += 2 * P.node_macs;
+    ctx->stats.leaf_tensors += P.leaves.size();
+  }
+  const int64_t nleaves = leaf_start[nb];
+  // layout of the upload (256-byte aligned arrays)
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = (off + 255) / 256 * 256;
+    off = o + bytes;
+    return o;
+  };
+  const size_t o_leaves = take((size_t)nleaves * sizeof(LeafDesc));
+  const size_t o_epi = take(nb * sizeof(EpiDesc));
+  const size_t o_res = take(4 * nb);
+  std::vector<size_t> o_desc(W), o_pfx(W), o_one(W);
+  for (int w = 0; w < W; w++) {
+    if (!cnt[w]) continue;
+    o_desc[w] = take((size_t)cnt[w] * desc_size(w / (H + 1)));
+    o_pfx[w] = take((size_t)(cnt[w] + 1) * sizeof(int64_t));
+    if (per_step) o_one[w] = take((size_t)cnt[w] * 2 * sizeof(int64_t));
+  }
+  const size_t bytes = off;
+  Staging& stg = (bidx & 1) ? ctx->staging2 : ctx->staging;
+  int rc = stg.ensure(bytes);
+  if (!rc) rc = ctx->results_pin[bidx & 1].ensure(4 * nb);
+  if (rc) return rc;
+  char* hb = static_cast<char*>(stg.host);
+  for (int w = 0; w < W; w++) {
+    if (!cnt[w]) continue;
+    std::memcpy(hb + o_pfx[w], pfx[w].data(), pfx[w].size() * sizeof(int64_t));
+    if (per_step) {
+      int64_t* one = reinterpret_cast<int64_t*>(hb + o_one[w]);
+      for (int32_t k = 0; k < cnt[w]; k++) {
+        one[2 * k] = 0;
+        one[2 * k + 1] = pfx[w][k + 1] - pfx[w][k];
+      }
+    }
+  }
+  // pass 2
+  std::vector<std::vector<int32_t>> local(std::max(1, ctx->plan_threads),
+                                          std::vector<int32_t>(W));
+  parallel_for(nb, ctx->plan_threads, [&](size_t i, int t) {
+    const SlicePlan& P = *plans[i];
+    const uint64_t b = sbase[i];
+    EpiDesc e;
+    std::memset(&e, 0, sizeof e);
+    e.rbits = add_r ? slices[i]->rbits : zero_bits;
+    e.root = P.empty ? 0 : b + (uint64_t)P.tensors[P.root].off;
+    std::memcpy(hb + o_epi + i * sizeof(EpiDesc), &e, sizeof e);
+    if (P.empty) return;
+    LeafDesc* ld = reinterpret_cast<LeafDesc*>(hb + o_leaves) + leaf_start[i];
+    for (const PLeaf& L : P.leaves) {
+      ld->p = b + (uint64_t)P.tensors[L.tensor].off;
+      ld->kind = L.kind;
+      ld->wbits = (int32_t)L.wbits;
+      ld++;
+    }
+    std::vector<int32_t>& loc = local[t];
+    std::fill(loc.begin(), loc.end(), 0);
+    const int32_t* st = &start[i * W];
+    for (const POp& op : P.ops) {
+      const int k = op_kind(op), w = slot(k, op.level);
+      const int32_t idx = st[w] + loc[w]++;
+      if (k >= 2)
+        reinterpret_cast<GemmDesc*>(hb + o_desc[w])[idx] = make_gemm(op, P.tensors, b);
+      else
+        reinterpret_cast<EwDesc*>(hb + o_desc[w])[idx] = make_ew(op, P.tensors, b);
+    }
+  });
+  ctx->stats.host_ms[1] += ms_since(t_pack);
+  // device descriptors: if the reserve cannot hold two batches' worth,
+  // wait for the batch in flight first
+  int64_t doff = ctx->pool.alloc(bytes);
+  if (doff < 0) {
+    if ((rc = finish(done_tag))) return rc;
+    doff = ctx->pool.alloc(bytes);
+  }
+  if (doff < 0) {
+    set_error("pool exhausted (descriptors)");
+    return TBH_ENOMEM;
+  }
+  auto t_issue = std::chrono::steady_clock::now();
+  char* dbase = ctx->base + doff;
+  hipError_t e = hipMemcpyAsync(dbase, hb, bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), nleaves, ctx->stream);
+  ctx->stats.launches += nleaves ? 1 : 0;
+  hipEvent_t ev = nullptr;
+  // one launch per kernel kind and level (direct EW, tiled EW, GEMM); in
+  // the per-step profiling mode every step alone, tagged with its shape
+  // GEMM launches on a second stream beside a level's EW launches (not
+  // while profiling): a level's steps are independent, so the VALU-bound
+  // GEMM tiles and the memory-bound EW tiles can share the CUs.  Each
+  // stream waits for the other's work of earlier levels (events recorded
+  // lazily, only when the other stream launched since the last sync).
+  const bool conc = ctx->tune.concurrent != 0 && !prof;
+  hipStream_t sk[2] = {ctx->stream, conc ? ctx->stream2 : ctx->stream};
+  auto& evpool = ctx->lvl_events[bidx & 1];
+  size_t evn = 0;
+  // dirty[s]: stream s launched work the other stream has not waited for
+  // (the upload and the leaves are stream-0 work)
+  bool dirty[2] = {true, false};
+  auto sync_to = [&](int s) {  // stream s waits for the other stream's work so far
+    const int o = 1 - s;
+    if (!conc || !dirty[o]) return hipSuccess;
+    if (evn == evpool.size()) {
+      hipEvent_t x;
+      hipError_t r = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+      if (r != hipSuccess) return r;
+      evpool.push_back(x);
+    }
+    hipEvent_t x = evpool[evn++];
+    hipError_t r = hipEventRecord(x, sk[o]);
+    if (r == hipSuccess) r = hipStreamWaitEvent(sk[s], x, 0);
+    dirty[o] = false;
+    return r;
+  };
+  auto launch = [&](int k, const char* descs, const int64_t* prefix, int nops, int64_t total) {
+    if (k == 0)
+      return launch_ew_direct(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
+                              direct_grid(ctx, total, direct_chunk(ctx, total)),
+                              direct_chunk(ctx, total), sk[0]);
+    if (k == 1)
+      return launch_ew(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
+                       ew_grid(ctx, dtype, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
+                       sk[0]);
+    return launch_gemm(dtype, true, k - 2, reinterpret_cast<const GemmDesc*>(descs), prefix, nops,
+                       total, gemm_grid(ctx, total, dtype, true, k - 2), sk[1]);
+  };
+  for (int L = 1; L <= H && e == hipSuccess; L++) {
+    if (conc) {
+      // a stream launching work of this level first waits for the other
+      // stream's work of earlier levels (both syncs before any launch)
+      bool has[2] = {false, false};
+      for (int k = 0; k < kKinds; k++)
+        if (cnt[slot(k, L)]) has[k >= 2 ? 1 : 0] = true;
+      if (has[0] && e == hipSuccess) e = sync_to(0);
+      if (has[1] && e == hipSuccess) e = sync_to(1);
+      if (has[0]) dirty[0] = true;
+      if (has[1]) dirty[1] = true;
+    }
+    for (int k = 0; k < kKinds && e == hipSuccess; k++) {
+      const int w = slot(k, L);
+      if (!cnt[w]) continue;
+      const int64_t total = pfx[w].back();
+      if (per_step) {
+        for (int32_t q = 0; q < cnt[w] && e == hipSuccess; q++) {
+          char tag_s[160];
+          if (k >= 2) {
+            const GemmDesc& d = reinterpret_cast<const GemmDesc*>(hb + o_desc[w])[q];
+            std::snprintf(tag_s, sizeof tag_s, "M=%d N=%d K=%d batch=%d", d.M, d.N, d.K, d.batch);
+          } else {
+            const EwDesc& d = reinterpret_cast<const EwDesc*>(hb + o_desc[w])[q];
+            if (k == 0 && d.nO < 8) continue;  // only the big direct steps
+            std::snprintf(tag_s, sizeof tag_s, "nT=%d nO=%d nR=%d nRo=%d nSA=%d nSB=%d op=%d",
+                          d.nT, d.nO, d.nR, d.nRo, d.nSA, d.nSB, d.mlay);
+          }
+          const int64_t tiles = pfx[w][q + 1] - pfx[w][q];
+          ev = ctx->prof.begin(ctx->stream);
+          e = launch(k, dbase + o_desc[w] + q * desc_size(k),
+                     reinterpret_cast<const int64_t*>(dbase + o_one[w]) + 2 * q, 1, tiles);
+          ctx->prof.end(ctx->stream, ev, stat_kind(k), L, 1, step_work[w][q].first,
+                        step_work[w][q].second, tag_s);
+          ctx->stats.launches++;
+        }
+        continue;
+      }
+      if (prof) ev = ctx->prof.begin(ctx->stream);
+      e = launch(k, dbase + o_desc[w], reinterpret_cast<const int64_t*>(dbase + o_pfx[w]), cnt[w],
+                 total);
+      if (prof) ctx->prof.end(ctx->stream, ev, stat_kind(k), L, cnt[w], lvl_bytes[w], lvl_macs[w]);
+      ctx->stats.launches++;
+    }
+    ctx->stats.waves++;
+  }
+  if (e == hipSuccess && conc) e = sync_to(0);  // the last GEMMs before the epilogue
+  if (e == hipSuccess) {
+    e = launch_epilogue(dtype, (const EpiDesc*)(dbase + o_epi), (int64_t)nb, dbase + o_res,
+                        ctx->stream);
+    ctx->stats.launches++;
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ctx->results_pin[bidx & 1].host, dbase + o_res, 4 * nb, hipMemcpyDeviceToHost,
+                       ctx->stream);
+  hipEvent_t done = evs_[bidx & 1];
+  if (e == hipSuccess && done) e = hipEventRecord(done, ctx->stream);
+  ctx->stats.host_ms[2] += ms_since(t_issue);
+  if (e != hipSuccess) {
+    sync_all();
+    ctx->pool.free(doff);
+    set_error(std::string("batched launch: ") + hipGetErrorString(e));
+    return TBH_EDEVICE;
+  }
+  // this batch runs while the previous one is waited for
+  if ((rc = finish(done_tag))) {
+    sync_all();
+    ctx->pool.free(doff);
+    return rc;
+  }
+  prev_.on = true;
+  prev_.done = done;
+  prev_.doff = doff;
+  prev_.parity = bidx & 1;
+  prev_.nb = nb;
+  prev_.dst = dst;
+  prev_.tag = tag;
+  return 0;
+}
+
 int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*>& slices,
                     const PlanOf& plan_of, bool add_r, uint32_t* results) {
   const size_t n = slices.size();
   if (n == 0) return 0;
-  const uint32_t zero_bits = dtype == TBH_F32 ? f2b(0.0f) : 0u;
-  const bool per_step = ctx->profiling && std::getenv("TBH_PROFILE_STEPS");
-  const bool prof = ctx->profiling;
-  int rc = ctx->results_host.ensure(4 * n);
-  if (rc) return rc;
-  uint32_t* res_host = static_cast<uint32_t*>(ctx->results_host.host);
-  // the arena region: the largest free block less a reserve for descriptors
-  const size_t largest = ctx->pool.largest_free();
-  const size_t reserve = std::min<size_t>(
-      (size_t)8 << 30, std::max(largest / 8, std::min<size_t>(largest / 4, (size_t)16 << 20)));
-  const size_t region = largest - reserve;
-  const int64_t region_off = region >= 256 ? ctx->pool.alloc(region) : -1;
-  if (region_off < 0) {
-    set_error("pool too small for a slice arena");
-    return TBH_ENOMEM;
-  }
-  const uint64_t arena_base = (uint64_t)(ctx->base + region_off);
-  struct Pending {
-    bool on = false;
-    hipEvent_t done = nullptr;
-    int64_t doff = -1;
-  } prev;
-  hipEvent_t evs[2] = {nullptr, nullptr};
-  for (auto& ev : evs)
-    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+  BatchRunner run(ctx);
+  int status = run.open();
+  if (status) return status;
+  const size_t region = run.region();
   std::string err;
-  int status = 0;
-  auto fail = [&](int code, const std::string& m) {
-    set_error(m);
-    status = code;
-  };
-  // waits for the batch in flight and releases its descriptors
-  auto finish = [&](Pending& p) {
-    if (!p.on) return true;
-    auto t_wait = std::chrono::steady_clock::now();
-    hipError_t e = p.done ? hipEventSynchronize(p.done) : hipStreamSynchronize(ctx->stream);
-    ctx->stats.host_ms[3] += ms_since(t_wait);
-    ctx->pool.free(p.doff);
-    p.on = false;
-    if (e != hipSuccess) {
-      fail(TBH_EDEVICE, std::string("batched execution: ") + hipGetErrorString(e));
-      return false;
-    }
-    return true;
-  };
   // the first batch is kept to a fraction of the region so that the GPU
   // starts while the later slices are still being planned
   const double first_frac = std::min(1.0, std::max(0.01, ctx->tune.first_batch));
@@ -566,278 +859,34 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     // admit slices while their arenas fit the region (waiting for plans)
     const size_t cap = bidx == 0 ? (size_t)(first_frac * (double)region) : region;
     std::vector<const SlicePlan*> plans;
+    std::vector<const SliceOwned*> sl;
     size_t i1 = i0, arena = 0;
     while (i1 < n) {
       const SlicePlan* P = plan_of(i1, &err);
       if (!P) {
-        fail(TBH_EINVAL, err);
+        set_error(err);
+        status = TBH_EINVAL;
         break;
       }
       const size_t a = P->empty ? 0 : (size_t)P->arena_bytes;
       if (i1 > i0 && arena + a > cap) break;
       if (a > region) {
-        fail(TBH_ENOMEM, "slice arena (" + std::to_string(a) + " bytes) does not fit the pool");
+        set_error("slice arena (" + std::to_string(a) + " bytes) does not fit the pool");
+        status = TBH_ENOMEM;
         break;
       }
       arena += a;
       plans.push_back(P);
+      sl.push_back(slices[i1]);
       i1++;
     }
     ctx->stats.host_ms[0] += ms_since(t_plan);
     if (status) break;
-    auto t_pack = std::chrono::steady_clock::now();
-    const size_t nb = i1 - i0;
-    std::vector<uint64_t> sbase(nb);
-    int H = 0;
-    {
-      uint64_t acc = 0;
-      for (size_t i = 0; i < nb; i++) {
-        sbase[i] = arena_base + acc;
-        acc += plans[i]->empty ? 0 : (uint64_t)plans[i]->arena_bytes;
-        H = std::max(H, plans[i]->nlevels);
-      }
-    }
-    const int W = kKinds * (H + 1);  // (kind, level) slots
-    auto slot = [H](int k, int L) { return k * (H + 1) + L; };
-    // pass 1
-    std::vector<int32_t> cnt(W, 0), start(nb * W);
-    std::vector<std::vector<int64_t>> pfx(W, std::vector<int64_t>(1, 0));
-    std::vector<double> lvl_bytes(W, 0.0), lvl_macs(W, 0.0);
-    std::vector<std::vector<std::pair<double, double>>> step_work(per_step ? W : 0);
-    std::vector<int64_t> leaf_start(nb + 1, 0);
-    for (size_t i = 0; i < nb; i++) {
-      const SlicePlan& P = *plans[i];
-      std::memcpy(&start[i * W], cnt.data(), W * sizeof(int32_t));
-      leaf_start[i + 1] = leaf_start[i] + (P.empty ? 0 : (int64_t)P.leaves.size());
-      if (P.empty) continue;
-      for (const POp& op : P.ops) {
-        const int w = slot(op_kind(op), op.level);
-        cnt[w]++;
-        pfx[w].push_back(pfx[w].back() + op_tiles(op));
-        lvl_bytes[w] += op.bytes;
-        lvl_macs[w] += op.macs;
-        if (per_step) step_work[w].push_back({op.bytes, op.macs});
-        if (op.kind == kEw) {
-          ctx->stats.ew_ops++;
-          ctx->stats.ew_ops_algo_bytes += op.bytes;
-        } else {
-          ctx->stats.gemm_ops++;
-          ctx->stats.gemm_ops_algo_ops += 2 * op.macs;
-        }
-      }
-      ctx->stats.total_algo_ops += 2 * P.node_macs;
-      ctx->stats.leaf_tensors += P.leaves.size();
-    }
-    const int64_t nleaves = leaf_start[nb];
-    // layout of the upload (256-byte aligned arrays)
-    size_t off = 0;
-    auto take = [&](size_t bytes) {
-      const size_t o = (off + 255) / 256 * 256;
-      off = o + bytes;
-      return o;
-    };
-    const size_t o_leaves = take((size_t)nleaves * sizeof(LeafDesc));
-    const size_t o_epi = take(nb * sizeof(EpiDesc));
-    const size_t o_res = take(4 * nb);
-    std::vector<size_t> o_desc(W), o_pfx(W), o_one(W);
-    for (int w = 0; w < W; w++) {
-      if (!cnt[w]) continue;
-      o_desc[w] = take((size_t)cnt[w] * desc_size(w / (H + 1)));
-      o_pfx[w] = take((size_t)(cnt[w] + 1) * sizeof(int64_t));
-      if (per_step) o_one[w] = take((size_t)cnt[w] * 2 * sizeof(int64_t));
-    }
-    const size_t bytes = off;
-    Staging& stg = (bidx & 1) ? ctx->staging2 : ctx->staging;
-    if ((rc = stg.ensure(bytes))) {
-      status = rc;
-      break;
-    }
-    char* hb = static_cast<char*>(stg.host);
-    for (int w = 0; w < W; w++) {
-      if (!cnt[w]) continue;
-      std::memcpy(hb + o_pfx[w], pfx[w].data(), pfx[w].size() * sizeof(int64_t));
-      if (per_step) {
-        int64_t* one = reinterpret_cast<int64_t*>(hb + o_one[w]);
-        for (int32_t k = 0; k < cnt[w]; k++) {
-          one[2 * k] = 0;
-          one[2 * k + 1] = pfx[w][k + 1] - pfx[w][k];
-        }
-      }
-    }
-    // pass 2
-    std::vector<std::vector<int32_t>> local(std::max(1, ctx->plan_threads),
-                                            std::vector<int32_t>(W));
-    parallel_for(nb, ctx->plan_threads, [&](size_t i, int t) {
-      const SlicePlan& P = *plans[i];
-      const uint64_t b = sbase[i];
-      EpiDesc e;
-      std::memset(&e, 0, sizeof e);
-      e.rbits = add_r ? slices[i0 + i]->rbits : zero_bits;
-      e.root = P.empty ? 0 : b + (uint64_t)P.tensors[P.root].off;
-      std::memcpy(hb + o_epi + i * sizeof(EpiDesc), &e, sizeof e);
-      if (P.empty) return;
-      LeafDesc* ld = reinterpret_cast<LeafDesc*>(hb + o_leaves) + leaf_start[i];
-      for (const PLeaf& L : P.leaves) {
-        ld->p = b + (uint64_t)P.tensors[L.tensor].off;
-        ld->kind = L.kind;
-        ld->wbits = (int32_t)L.wbits;
-        ld++;
-      }
-      std::vector<int32_t>& loc = local[t];
-      std::fill(loc.begin(), loc.end(), 0);
-      const int32_t* st = &start[i * W];
-      for (const POp& op : P.ops) {
-        const int k = op_kind(op), w = slot(k, op.level);
-        const int32_t idx = st[w] + loc[w]++;
-        if (k >= 2)
-          reinterpret_cast<GemmDesc*>(hb + o_desc[w])[idx] = make_gemm(op, P.tensors, b);
-        else
-          reinterpret_cast<EwDesc*>(hb + o_desc[w])[idx] = make_ew(op, P.tensors, b);
-      }
-    });
-    ctx->stats.host_ms[1] += ms_since(t_pack);
-    // device descriptors: if the reserve cannot hold two batches' worth,
-    // wait for the batch in flight first
-    int64_t doff = ctx->pool.alloc(bytes);
-    if (doff < 0) {
-      if (!finish(prev)) break;
-      doff = ctx->pool.alloc(bytes);
-    }
-    if (doff < 0) {
-      fail(TBH_ENOMEM, "pool exhausted (descriptors)");
-      break;
-    }
-    auto t_issue = std::chrono::steady_clock::now();
-    char* dbase = ctx->base + doff;
-    hipError_t e = hipMemcpyAsync(dbase, hb, bytes, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess)
-      e = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), nleaves, ctx->stream);
-    ctx->stats.launches += nleaves ? 1 : 0;
-    hipEvent_t ev = nullptr;
-    // one launch per kernel kind and level (direct EW, tiled EW, GEMM); in
-    // the per-step profiling mode every step alone, tagged with its shape
-    // GEMM launches on a second stream beside a level's EW launches (not
-    // while profiling): a level's steps are independent, so the VALU-bound
-    // GEMM tiles and the memory-bound EW tiles can share the CUs.  Each
-    // stream waits for the other's work of earlier levels (events recorded
-    // lazily, only when the other stream launched since the last sync).
-    const bool conc = ctx->tune.concurrent != 0 && !prof;
-    hipStream_t sk[2] = {ctx->stream, conc ? ctx->stream2 : ctx->stream};
-    auto& evpool = ctx->lvl_events[bidx & 1];
-    size_t evn = 0;
-    // dirty[s]: stream s launched work the other stream has not waited for
-    // (the upload and the leaves are stream-0 work)
-    bool dirty[2] = {true, false};
-    auto sync_to = [&](int s) {  // stream s waits for the other stream's work so far
-      const int o = 1 - s;
-      if (!conc || !dirty[o]) return hipSuccess;
-      if (evn == evpool.size()) {
-        hipEvent_t x;
-        hipError_t r = hipEventCreateWithFlags(&x, hipEventDisableTiming);
-        if (r != hipSuccess) return r;
-        evpool.push_back(x);
-      }
-      hipEvent_t x = evpool[evn++];
-      hipError_t r = hipEventRecord(x, sk[o]);
-      if (r == hipSuccess) r = hipStreamWaitEvent(sk[s], x, 0);
-      dirty[o] = false;
-      return r;
-    };
-    auto launch = [&](int k, const char* descs, const int64_t* prefix, int nops, int64_t total) {
-      if (k == 0)
-        return launch_ew_direct(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
-                                direct_grid(ctx, total, direct_chunk(ctx, total)),
-                                direct_chunk(ctx, total), sk[0]);
-      if (k == 1)
-        return launch_ew(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
-                         ew_grid(ctx, dtype, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
-                         sk[0]);
-      return launch_gemm(dtype, true, k - 2, reinterpret_cast<const GemmDesc*>(descs), prefix,
-                         nops, total, gemm_grid(ctx, total, dtype, true, k - 2), sk[1]);
-    };
-    for (int L = 1; L <= H && e == hipSuccess; L++) {
-      if (conc) {
-        // a stream launching work of this level first waits for the other
-        // stream's work of earlier levels (both syncs before any launch)
-        bool has[2] = {false, false};
-        for (int k = 0; k < kKinds; k++)
-          if (cnt[slot(k, L)]) has[k >= 2 ? 1 : 0] = true;
-        if (has[0] && e == hipSuccess) e = sync_to(0);
-        if (has[1] && e == hipSuccess) e = sync_to(1);
-        if (has[0]) dirty[0] = true;
-        if (has[1]) dirty[1] = true;
-      }
-      for (int k = 0; k < kKinds && e == hipSuccess; k++) {
-        const int w = slot(k, L);
-        if (!cnt[w]) continue;
-        const int64_t total = pfx[w].back();
-        if (per_step) {
-          for (int32_t q = 0; q < cnt[w] && e == hipSuccess; q++) {
-            char tag[160];
-            if (k >= 2) {
-              const GemmDesc& d = reinterpret_cast<const GemmDesc*>(hb + o_desc[w])[q];
-              std::snprintf(tag, sizeof tag, "M=%d N=%d K=%d batch=%d", d.M, d.N, d.K, d.batch);
-            } else {
-              const EwDesc& d = reinterpret_cast<const EwDesc*>(hb + o_desc[w])[q];
-              if (k == 0 && d.nO < 8) continue;  // only the big direct steps
-              std::snprintf(tag, sizeof tag, "nT=%d nO=%d nR=%d nRo=%d nSA=%d nSB=%d op=%d", d.nT,
-                            d.nO, d.nR, d.nRo, d.nSA, d.nSB, d.mlay);
-            }
-            const int64_t tiles = pfx[w][q + 1] - pfx[w][q];
-            ev = ctx->prof.begin(ctx->stream);
-            e = launch(k, dbase + o_desc[w] + q * desc_size(k),
-                       reinterpret_cast<const int64_t*>(dbase + o_one[w]) + 2 * q, 1, tiles);
-            ctx->prof.end(ctx->stream, ev, stat_kind(k), L, 1, step_work[w][q].first,
-                          step_work[w][q].second, tag);
-            ctx->stats.launches++;
-          }
-          continue;
-        }
-        if (prof) ev = ctx->prof.begin(ctx->stream);
-        e = launch(k, dbase + o_desc[w], reinterpret_cast<const int64_t*>(dbase + o_pfx[w]),
-                   cnt[w], total);
-        if (prof) ctx->prof.end(ctx->stream, ev, stat_kind(k), L, cnt[w], lvl_bytes[w], lvl_macs[w]);
-        ctx->stats.launches++;
-      }
-      ctx->stats.waves++;
-    }
-    if (e == hipSuccess && conc) e = sync_to(0);  // the last GEMMs before the epilogue
-    if (e == hipSuccess) {
-      e = launch_epilogue(dtype, (const EpiDesc*)(dbase + o_epi), (int64_t)nb, dbase + o_res,
-                          ctx->stream);
-      ctx->stats.launches++;
-    }
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(res_host + i0, dbase + o_res, 4 * nb, hipMemcpyDeviceToHost, ctx->stream);
-    hipEvent_t done = evs[bidx & 1];
-    if (e == hipSuccess && done) e = hipEventRecord(done, ctx->stream);
-    ctx->stats.host_ms[2] += ms_since(t_issue);
-    if (e != hipSuccess) {
-      sync_both(ctx);
-      ctx->pool.free(doff);
-      fail(TBH_EDEVICE, std::string("batched launch: ") + hipGetErrorString(e));
-      break;
-    }
-    // the next batch is packed while this one runs; the previous is done
-    if (!finish(prev)) {
-      sync_both(ctx);
-      ctx->pool.free(doff);
-      break;
-    }
-    prev.on = true;
-    prev.done = done;
-    prev.doff = doff;
+    status = run.enqueue(dtype, sl, plans, add_r, results + i0, nullptr, nullptr);
     i0 = i1;
   }
-  if (prev.on && !finish(prev) && status == 0) status = TBH_EDEVICE;
-  if (status) sync_both(ctx);
-  for (auto& ev : evs)
-    if (ev) (void)hipEventDestroy(ev);
-  ctx->pool.free(region_off);
-  if (prof) ctx->prof.collect(ctx->stats);
-  if (status) return status;
-  std::memcpy(results, res_host, 4 * n);
-  return 0;
+  if (status == 0) status = run.flush(nullptr);
+  return status;
 }
 
 // -------------------------------------------------------- PLAIN ----------

@@ -99,7 +99,7 @@ struct tbh_ctx {
   char* base = nullptr;  // device pool
   tbh::Pool pool;
   tbh::Staging staging, staging2;  // descriptor uploads (ping-pong across batches)
-  tbh::Staging results_host;       // per-slice results read back
+  tbh::Staging results_pin[2];     // per-batch results read back (by batch parity)
   std::mutex mu;         // one executor call at a time
   tbh_stats stats{};
   int plan_threads = 8;
@@ -142,6 +142,53 @@ class PlanStream {
   size_t bad_ = (size_t)-1;
   std::string bad_err_;
   std::vector<std::thread> th_;
+};
+
+// The arena region a batched run takes when `largest` bytes are the
+// largest free block of the pool (the rest stays for step descriptors).
+size_t region_bytes(size_t largest);
+
+// Runs batches of planned slices through one arena region of the pool, one
+// batch in flight behind the one being enqueued.  enqueue() packs a batch
+// (its arenas must fit region() together), launches it, then waits for the
+// batch enqueued before it, whose results it copies to that batch's `dst`
+// and whose `tag` it returns in *done_tag (nullptr: none).  flush() waits
+// for the last batch the same way.  After an error the results of the
+// batch being enqueued and of the one in flight are lost; close() (also the
+// destructor) waits for the streams and releases the region.  The caller
+// holds ctx->mu from open() to close().
+class BatchRunner {
+ public:
+  explicit BatchRunner(tbh_ctx* ctx);
+  ~BatchRunner();
+  int open();
+  void close();
+  size_t region() const { return region_; }
+  bool busy() const { return prev_.on; }
+  int enqueue(int dtype, const std::vector<const SliceOwned*>& slices,
+              const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* dst, void* tag,
+              void** done_tag);
+  int flush(void** done_tag);
+
+ private:
+  int finish(void** done_tag);
+  void sync_all();
+  struct Pending {
+    bool on = false;
+    hipEvent_t done = nullptr;
+    int64_t doff = -1;
+    int parity = 0;
+    size_t nb = 0;
+    uint32_t* dst = nullptr;
+    void* tag = nullptr;
+  };
+  tbh_ctx* ctx_;
+  int64_t region_off_ = -1;
+  size_t region_ = 0;
+  uint64_t arena_base_ = 0;
+  int bidx_ = 0;
+  hipEvent_t evs_[2] = {nullptr, nullptr};
+  Pending prev_;
 };
 
 // Executes the plans of `slices` (plan_of(i) may block until slice i is

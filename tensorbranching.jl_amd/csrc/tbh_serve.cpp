@@ -5,18 +5,26 @@
 // of (slice optimum + r), as maximum(contract_slices(...)) in
 // src/dynamic_ob.jl:24-27.  Requests flow through two threads:
 //   planner    plans each submitted request's slices (BATCHED plans, on the
-//              context's plan threads) while the GPU runs earlier rounds;
-//   scheduler  forms rounds from the planned requests in arrival order --
-//              latency-lane requests (priority 1) ahead of the others, then
-//              the unscheduled slices of the oldest request first, then the
-//              next request's, up to max_resident_slices (and one dtype per
-//              round) -- runs each round as one BATCHED execution (its waves
-//              mix slices of every request in the round), and folds the
-//              per-slice results into their requests.  A request joins the
-//              next round after it is planned, so requests of any size are
-//              admitted while others are in flight.
-// Each slice's plan, and so its result, does not depend on the other slices
-// of its round: a request's value is the same served alone or mixed.
+//              context's plan threads) while the GPU runs earlier rounds, and
+//              refuses a request alone when one of its slices' arenas cannot
+//              fit the serving region of the pool;
+//   scheduler  forms rounds from the planned requests -- latency-lane
+//              requests (priority 1) first, then the others in arrival order,
+//              each request's unscheduled slices in order -- and admits slices
+//              while the round's summed arena footprint fits the region, its
+//              summed estimated work (MACs of the slices' trees) stays within
+//              the round bound, and at most max_resident_slices are in it (one
+//              dtype per round).  Each round is one batched execution whose
+//              waves mix slices of every request in it.  Rounds are pipelined:
+//              round k+1 is packed and enqueued on the stream while round k
+//              runs, so the GPU does not wait for the host between rounds, and
+//              a request that arrives meanwhile (or a latency-lane slice)
+//              waits for at most the round running and the one queued, each
+//              bounded by the round work.
+// A round that fails is run again request by request; only the requests
+// that fail alone end FAILED.  Each slice's plan, and so its result, does not
+// depend on the other slices of its round: a request's value is the same
+// served alone or mixed.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -49,6 +57,13 @@ struct Req {
   double t_submit = 0, t_done = 0;
 };
 
+// One batched execution: (request, slice index) pairs and their results.
+struct Round {
+  int dtype = TBH_F32;
+  std::vector<std::pair<std::shared_ptr<Req>, size_t>> items;
+  std::vector<uint32_t> res;
+};
+
 // larger of two dtype values given as bit patterns
 bool greater(int dtype, uint32_t a, uint32_t b) {
   if (dtype == TBH_F32) {
@@ -60,11 +75,15 @@ bool greater(int dtype, uint32_t a, uint32_t b) {
   return (int32_t)a > (int32_t)b;
 }
 
+size_t arena_of(const SlicePlan& P) { return P.empty ? 0 : (size_t)P.arena_bytes; }
+
 }  // namespace
 
 struct ServeState {
   tbh_ctx* ctx = nullptr;
   size_t max_resident = 0;
+  double round_macs = 0;    // estimated work bound of a round
+  size_t region_cap = 0;    // arena region of an idle pool: a slice's upper bound
   std::mutex m;
   std::condition_variable cv_plan, cv_sched, cv_done;
   std::deque<std::shared_ptr<Req>> to_plan;  // submitted, not planned
@@ -73,11 +92,16 @@ struct ServeState {
   uint64_t next_ticket = 1;
   bool stop = false;
   std::thread planner, scheduler;
+  // counters (tbh_serve_stats)
+  uint64_t rounds = 0, retried_rounds = 0, failed_requests = 0;
 
   void fail(const std::shared_ptr<Req>& r, const std::string& why) {
+    if (r->status == TBH_DONE || r->status == TBH_FAILED) return;
     r->status = TBH_FAILED;
     r->err = why;
     r->t_done = now_s();
+    failed_requests++;
+    ready.erase(std::remove(ready.begin(), ready.end(), r), ready.end());
   }
 
   void plan_loop() {
@@ -95,11 +119,20 @@ struct ServeState {
       std::vector<SlicePlan> plans;
       int rc = plan_all(ctx, ptrs, true, &plans);
       std::string err = rc ? std::string(tbh_last_error()) : std::string();
+      // admission by footprint: every slice's arena must fit the region an
+      // idle pool gives a round, or the request is refused alone
+      for (size_t i = 0; !rc && i < plans.size(); i++)
+        if (arena_of(plans[i]) > region_cap) {
+          rc = TBH_ENOMEM;
+          err = "slice " + std::to_string(i) + ": arena of " +
+                std::to_string(arena_of(plans[i])) + " bytes exceeds the serving region (" +
+                std::to_string(region_cap) + " bytes of the pool)";
+        }
       {
         std::lock_guard<std::mutex> g(m);
         if (r->status == TBH_FAILED) continue;  // cancelled while planning
         if (rc) {
-          fail(r, "planning: " + err);
+          fail(r, (rc == TBH_ENOMEM ? "admission: " : "planning: ") + err);
           cv_done.notify_all();
           continue;
         }
@@ -110,77 +143,191 @@ struct ServeState {
     }
   }
 
+  // Takes slices for the next round (caller holds m); empty when nothing
+  // is ready.  Slices are taken in order -- latency lane first, then arrival
+  // order, each request's slices in index order -- until the next one would
+  // break a bound; the first slice is always taken.
+  std::shared_ptr<Round> form_round(size_t region) {
+    auto R = std::make_shared<Round>();
+    if (ready.empty()) return R;
+    std::vector<std::shared_ptr<Req>> order;
+    for (auto& r : ready)
+      if (r->priority) order.push_back(r);
+    for (auto& r : ready)
+      if (!r->priority) order.push_back(r);
+    R->dtype = order.front()->dtype;
+    const size_t cap = max_resident ? max_resident : (size_t)-1;
+    size_t bytes = 0;
+    double work = 0;
+    bool full = false;
+    for (auto& r : order) {
+      if (r->dtype != R->dtype) continue;
+      while (r->next < r->slices.size()) {
+        const SlicePlan& P = r->plans[r->next];
+        const size_t a = arena_of(P);
+        const double w = P.empty ? 0.0 : P.node_macs;
+        if (!R->items.empty() &&
+            (R->items.size() >= cap || bytes + a > region || work + w > round_macs)) {
+          full = true;
+          break;
+        }
+        bytes += a;
+        work += w;
+        R->items.push_back({r, r->next++});
+        r->admitted = true;
+        r->status = TBH_RUNNING;
+      }
+      if (full) break;
+    }
+    // requests with every slice scheduled leave the ready list
+    ready.erase(std::remove_if(ready.begin(), ready.end(),
+                               [](const std::shared_ptr<Req>& r) {
+                                 return r->next >= r->slices.size();
+                               }),
+                ready.end());
+    R->res.assign(R->items.size(), 0);
+    return R;
+  }
+
+  // Folds a finished round's results into its requests (caller holds m).
+  void fold(const Round& R) {
+    const double t = now_s();
+    for (size_t i = 0; i < R.items.size(); i++) {
+      Req& r = *R.items[i].first;
+      if (r.status == TBH_FAILED) continue;
+      if (!r.has_best || greater(R.dtype, R.res[i], r.best)) r.best = R.res[i];
+      r.has_best = true;
+      if (++r.done == r.slices.size()) {
+        r.status = TBH_DONE;
+        r.t_done = t;
+      }
+    }
+  }
+
+  // After a failed execution: each request's slices of the round run again
+  // on their own (the caller holds ctx->mu, no batched run open); a request
+  // that fails alone fails, the others are folded.
+  void rerun_alone(const Round& R) {
+    std::vector<std::shared_ptr<Req>> reqs;
+    for (auto& it : R.items)
+      if (std::find(reqs.begin(), reqs.end(), it.first) == reqs.end()) reqs.push_back(it.first);
+    for (auto& rq : reqs) {
+      Round one;
+      one.dtype = R.dtype;
+      for (auto& it : R.items)
+        if (it.first == rq) one.items.push_back(it);
+      one.res.assign(one.items.size(), 0);
+      std::vector<const SliceOwned*> sl(one.items.size());
+      std::vector<const SlicePlan*> pl(one.items.size());
+      for (size_t i = 0; i < one.items.size(); i++) {
+        sl[i] = &rq->slices[one.items[i].second];
+        pl[i] = &rq->plans[one.items[i].second];
+      }
+      const int rc = execute_batched(ctx, R.dtype, sl,
+                                     [&](size_t i, std::string*) { return pl[i]; }, true,
+                                     one.res.data());
+      const std::string err = rc ? std::string(tbh_last_error()) : std::string();
+      std::lock_guard<std::mutex> g(m);
+      if (rc)
+        fail(rq, "execution: " + err);
+      else
+        fold(one);
+    }
+  }
+
   void sched_loop() {
     (void)hipSetDevice(ctx->device);
+    BatchRunner run(ctx);
+    std::unique_lock<std::mutex> held(ctx->mu, std::defer_lock);
+    std::shared_ptr<Round> inflight;
+    // ends the batched run and lets synchronous callers of the context in
+    auto idle = [&] {
+      run.close();
+      if (held.owns_lock()) held.unlock();
+    };
+    auto finish_inflight = [&] {
+      if (!inflight) return;
+      void* done = nullptr;
+      const int rc = run.flush(&done);
+      if (rc) {
+        run.close();
+        {
+          std::lock_guard<std::mutex> g(m);
+          retried_rounds++;
+        }
+        rerun_alone(*inflight);
+      } else {
+        std::lock_guard<std::mutex> g(m);
+        fold(*inflight);
+      }
+      inflight.reset();
+      cv_done.notify_all();
+    };
     for (;;) {
-      // form a round
-      std::vector<std::pair<std::shared_ptr<Req>, size_t>> round;
-      int dtype = TBH_F32;
+      bool stopping = false;
       {
         std::unique_lock<std::mutex> g(m);
-        cv_sched.wait(g, [&] { return stop || !ready.empty(); });
-        if (stop) return;
-        // latency-lane requests first (in arrival order), then the rest;
-        // the round's dtype is that of its first request
-        std::vector<std::shared_ptr<Req>> order;
-        for (auto& r : ready)
-          if (r->priority) order.push_back(r);
-        for (auto& r : ready)
-          if (!r->priority) order.push_back(r);
-        dtype = order.front()->dtype;
-        const size_t cap = max_resident ? max_resident : (size_t)-1;
-        for (auto& r : order) {
-          if (r->dtype != dtype) continue;
-          while (r->next < r->slices.size() && round.size() < cap) {
-            round.push_back({r, r->next++});
-            r->admitted = true;
-            r->status = TBH_RUNNING;
-          }
-          if (round.size() >= cap) break;
-        }
-        // requests with every slice scheduled leave the ready list
-        ready.erase(std::remove_if(ready.begin(), ready.end(),
-                                   [](const std::shared_ptr<Req>& r) {
-                                     return r->next >= r->slices.size();
-                                   }),
-                    ready.end());
+        if (!inflight) cv_sched.wait(g, [&] { return stop || !ready.empty(); });
+        stopping = stop;
       }
-      // run it
-      std::vector<const SliceOwned*> sl(round.size());
-      std::vector<const SlicePlan*> pl(round.size());
-      for (size_t i = 0; i < round.size(); i++) {
-        sl[i] = &round[i].first->slices[round[i].second];
-        pl[i] = &round[i].first->plans[round[i].second];
+      if (stopping) {
+        finish_inflight();
+        idle();
+        return;
       }
-      std::vector<uint32_t> res(round.size());
-      int rc;
-      {
-        std::lock_guard<std::mutex> g(ctx->mu);
-        rc = execute_batched(ctx, dtype, sl, [&](size_t i, std::string*) { return pl[i]; }, true,
-                             res.data());
+      if (!held.owns_lock()) {
+        held.lock();
+        (void)hipSetDevice(ctx->device);
       }
-      const std::string err = rc ? std::string(tbh_last_error()) : std::string();
-      // fold results
+      if (run.region() == 0 && run.open()) {
+        // no region (the pool is taken): fail what is ready, one round's worth
+        const std::string err = tbh_last_error();
+        idle();
+        std::lock_guard<std::mutex> g(m);
+        auto R = form_round((size_t)-1);
+        for (auto& it : R->items) fail(it.first, "execution: " + err);
+        cv_done.notify_all();
+        continue;
+      }
+      std::shared_ptr<Round> R;
       {
         std::lock_guard<std::mutex> g(m);
-        const double t = now_s();
-        for (size_t i = 0; i < round.size(); i++) {
-          Req& r = *round[i].first;
-          if (r.status == TBH_FAILED) continue;
-          if (rc) {
-            fail(round[i].first, "execution: " + err);
-            ready.erase(std::remove(ready.begin(), ready.end(), round[i].first), ready.end());
-            continue;
-          }
-          if (!r.has_best || greater(dtype, res[i], r.best)) r.best = res[i];
-          r.has_best = true;
-          if (++r.done == r.slices.size()) {
-            r.status = TBH_DONE;
-            r.t_done = t;
-          }
-        }
+        R = form_round(run.region());
+        if (!R->items.empty()) rounds++;
       }
-      cv_done.notify_all();
+      if (R->items.empty()) {  // nothing new: finish the round in flight, go idle
+        finish_inflight();
+        idle();
+        continue;
+      }
+      std::vector<const SliceOwned*> sl(R->items.size());
+      std::vector<const SlicePlan*> pl(R->items.size());
+      for (size_t i = 0; i < R->items.size(); i++) {
+        sl[i] = &R->items[i].first->slices[R->items[i].second];
+        pl[i] = &R->items[i].first->plans[R->items[i].second];
+      }
+      void* done = nullptr;
+      const int rc = run.enqueue(R->dtype, sl, pl, true, R->res.data(), R.get(), &done);
+      if (rc) {
+        // neither this round's results nor (unless already handed back)
+        // those of the round in flight can be trusted: run them again
+        run.close();
+        {
+          std::lock_guard<std::mutex> g(m);
+          retried_rounds++;
+        }
+        if (inflight) rerun_alone(*inflight);
+        rerun_alone(*R);
+        inflight.reset();
+        cv_done.notify_all();
+        continue;
+      }
+      if (done) {
+        std::lock_guard<std::mutex> g(m);
+        fold(*inflight);
+      }
+      if (done) cv_done.notify_all();
+      inflight = R;
     }
   }
 };
@@ -202,9 +349,13 @@ static std::shared_ptr<ServeState> serving(tbh_ctx* ctx) {
 
 extern "C" {
 
-int tbh_serve_start(tbh_ctx* ctx, size_t max_resident_slices) {
-  if (!ctx) {
-    set_error("null context");
+int tbh_serve_start_ex(tbh_ctx* ctx, const tbh_serve_config* cfg) {
+  if (!ctx || !cfg) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  if (cfg->round_log2_macs != 0 && !(cfg->round_log2_macs >= 10 && cfg->round_log2_macs <= 60)) {
+    set_error("round_log2_macs must be 0 (default) or in [10, 60]");
     return TBH_EINVAL;
   }
   std::lock_guard<std::mutex> g(ctx->serve_mu);
@@ -214,10 +365,30 @@ int tbh_serve_start(tbh_ctx* ctx, size_t max_resident_slices) {
   }
   auto s = std::make_shared<ServeState>();
   s->ctx = ctx;
-  s->max_resident = max_resident_slices;
+  s->max_resident = cfg->max_resident_slices;
+  s->round_macs =
+      std::pow(2.0, cfg->round_log2_macs ? cfg->round_log2_macs : TBH_SERVE_ROUND_LOG2_MACS);
+  s->region_cap = tbh::region_bytes(ctx->pool.capacity());
   s->planner = std::thread([p = s.get()] { p->plan_loop(); });
   s->scheduler = std::thread([p = s.get()] { p->sched_loop(); });
   ctx->serve = s;
+  return 0;
+}
+
+int tbh_serve_start(tbh_ctx* ctx, size_t max_resident_slices) {
+  tbh_serve_config cfg{};
+  cfg.max_resident_slices = max_resident_slices;
+  return tbh_serve_start_ex(ctx, &cfg);
+}
+
+int tbh_serve_stats(tbh_ctx* ctx, uint64_t* rounds, uint64_t* retried_rounds,
+                    uint64_t* failed_requests) {
+  auto s = serving(ctx);
+  if (!s) return TBH_EINVAL;
+  std::lock_guard<std::mutex> g(s->m);
+  if (rounds) *rounds = s->rounds;
+  if (retried_rounds) *retried_rounds = s->retried_rounds;
+  if (failed_requests) *failed_requests = s->failed_requests;
   return 0;
 }
 

@@ -853,6 +853,17 @@ tbw_sliceset* tbw_whole(const tbw_graph* g, const double* weights,
 tbw_sliceset* tbw_slice(const tbw_graph* g, const double* weights,
                         int32_t sc_target, int32_t order_trials, uint64_t seed,
                         int32_t nthreads, int32_t do_kernel) {
+  return tbw_slice_limited(g, weights, sc_target, order_trials, seed, nthreads, do_kernel,
+                           (int64_t)1 << 18);
+}
+
+tbw_sliceset* tbw_slice_limited(const tbw_graph* g, const double* weights,
+                                int32_t sc_target, int32_t order_trials, uint64_t seed,
+                                int32_t nthreads, int32_t do_kernel, int64_t max_slices) {
+  if (max_slices < 1) {
+    g_err = "slice: max_slices must be positive";
+    return nullptr;
+  }
   if (nthreads <= 0) nthreads = (int32_t)std::max(1u, std::thread::hardware_concurrency());
   Branch root;
   root.nv = g->nv;
@@ -864,7 +875,7 @@ tbw_sliceset* tbw_slice(const tbw_graph* g, const double* weights,
   auto* ss = new tbw_sliceset;
   std::vector<Branch> frontier;
   frontier.push_back(std::move(root));
-  const size_t kMaxSlices = (size_t)1 << 18;
+  const size_t kMaxSlices = (size_t)max_slices;
   uint64_t round_seed = seed * 0x9E3779B97F4A7C15ULL + 1;
   // the root uses the tree searched above; later rounds refine briefly
   bool first = true;
@@ -893,7 +904,7 @@ tbw_sliceset* tbw_slice(const tbw_graph* g, const double* weights,
         for (auto& c : r.children) nf.push_back(std::move(c));
     }
     if (ss->items.size() + nf.size() > kMaxSlices) {
-      g_err = "slice: more than 2^18 slices; raise sc_target";
+      g_err = "slice: more than " + std::to_string(kMaxSlices) + " slices; raise sc_target";
       delete ss;
       return nullptr;
     }

@@ -10,14 +10,25 @@ from .api import _dtype_code, _Marshal
 
 
 class Server:
-    """Requests (lists of SlicedBranch) are admitted by device-memory
-    footprint as resident slices finish; results are the MIS values."""
+    """Requests (lists of SlicedBranch) are served in pipelined rounds: a
+    round admits slices while their summed arena footprint fits the pool's
+    region, their estimated work stays within 2^round_log2_macs MACs (0: the
+    library default) and at most max_resident_slices (0: no cap) are in it.
+    A request with a slice too large for the pool is refused alone.
+    Results are the MIS values."""
 
-    def __init__(self, backend, max_resident_slices=0):
+    def __init__(self, backend, max_resident_slices=0, round_log2_macs=0.0):
         self.be = backend
-        check(_lib.tbh().tbh_serve_start(backend.handle, int(max_resident_slices)))
+        cfg = _lib.TbhServeConfig(int(max_resident_slices), float(round_log2_macs))
+        check(_lib.tbh().tbh_serve_start_ex(backend.handle, C.byref(cfg)))
         self._open = True
         self._dt = {}
+
+    def stats(self):
+        """{'rounds', 'retried_rounds', 'failed_requests'} since start."""
+        v = [C.c_uint64() for _ in range(3)]
+        check(_lib.tbh().tbh_serve_stats(self.be.handle, *(C.byref(x) for x in v)))
+        return dict(zip(("rounds", "retried_rounds", "failed_requests"), (x.value for x in v)))
 
     def submit(self, branches, element_type=np.float32, priority=0):
         """priority=1: the latency lane (scheduled ahead of normal requests)."""

@@ -13,7 +13,7 @@ import numpy as np
 
 from . import _lib
 from .api import (CompressedEinsum, MISProblem, SimpleGraph, SlicedBranch, UnitWeight,
-                  contract_slices)
+                  complexity, contract_slices)
 
 
 def _tbw_err():
@@ -119,15 +119,17 @@ class ContractionTreeSlicer:
         self.kernelize = bool(kernelize)
 
 
-def slice_graph(g, weights=None, slicer=None, seed=0, nthreads=0):
+def slice_graph(g, weights=None, slicer=None, seed=0, nthreads=0, max_slices=1 << 18):
     """Kernelize, order and slice g until every slice has sc <= sc_target;
-    returns the SlicedBranch list whose maximum (+r) is the MIS."""
+    returns the SlicedBranch list whose maximum (+r) is the MIS.  Raises
+    RuntimeError as soon as the slices would outnumber max_slices."""
     slicer = slicer or ContractionTreeSlicer()
     wd, wp, wdt = _weights_arg(g, weights)
     h = _handle_of(g)
     try:
-        ss = _lib.tbw().tbw_slice(h, wp, slicer.sc_target, slicer.order_trials, seed, nthreads,
-                                  1 if slicer.kernelize else 0)
+        ss = _lib.tbw().tbw_slice_limited(h, wp, slicer.sc_target, slicer.order_trials, seed,
+                                          nthreads, 1 if slicer.kernelize else 0,
+                                          int(max_slices))
     finally:
         _lib.tbw().tbw_graph_free(h)
     if not ss:
@@ -144,26 +146,72 @@ def dynamic_ob_mis(g, weights=None, *, slicer=None, usecuda=True, element_type=n
     return res.max()
 
 
-def serving_requests(count, seed=0, order_trials=50):
-    """A mixed stream of MIS requests for the serving scheduler: random
-    3-regular graphs with even n ~ U[100, 260] and Erdos-Renyi graphs with n ~
-    U[100, 250], p = 3 / (n - 1), each sliced to sc_target ~ U[lo(n), 25]
-    with lo(n) = max(15, n // 10 - 3) (lower targets multiply the slice count
-    of large graphs into the 10^5 range).  Returns a list of
-    (description, graph, branches)."""
+#: the serving configuration's request ranges
+SERVING_MIX = ("3-regular n ~ U[100,400] (even) and Erdos-Renyi n ~ U[100,250] with "
+               "p = 3/(n-1), half each; sc_target ~ U[15,25]; half Tropical Float32 with unit "
+               "weights, half Tropical Int32 with integer weights ~ U[1,100]")
+
+
+class ServingRequest:
+    """One drawn MIS request: its graph, weights, element type and slices."""
+
+    def __init__(self, kind, n, gseed, sc_target, dtype, graph, weights, branches):
+        self.kind, self.n, self.gseed, self.sc_target = kind, n, gseed, sc_target
+        self.dtype, self.graph, self.weights, self.branches = dtype, graph, weights, branches
+        tcs = [complexity(b).tc for b in branches if b.code is not None]
+        self.log2_macs = float(np.log2(np.sum(np.exp2(tcs)))) if tcs else 0.0
+        self.desc = (f"{kind} n={n} seed={gseed} sc_target={sc_target} "
+                     f"{'f32' if dtype == np.float32 else 'i32'}")
+
+    def __iter__(self):  # (description, graph, branches)
+        return iter((self.desc, self.graph, self.branches))
+
+
+def serving_requests(count, seed=0, order_trials=50, max_slices=4096, max_log2_macs=None,
+                     max_width_gap=16, refused=None):
+    """A seeded stream of `count` MIS requests drawn from the serving
+    configuration's ranges (SERVING_MIX).  A draw the host slicer cannot
+    serve is refused and the next one drawn: (a) when the graph's unsliced
+    contraction width (a 20-trial tree of the unweighted graph) exceeds
+    sc_target by more than max_width_gap -- a cheap test that refuses most
+    3-regular graphs with n above about 300, whose slicing to sc_target <= 25
+    goes beyond 2^13 slices (n = 400 beyond 2^18); (b) when slicing yields
+    more than max_slices slices; (c) when max_log2_macs is given and the
+    request's work (log2 of the summed 2^tc of its slices) exceeds it.
+    `refused`, when a dict, receives the counts per reason.  Returns a list
+    of ServingRequest (iterable as (description, graph, branches))."""
     rng = np.random.default_rng(seed)
     out = []
-    for _ in range(count):
+    counts = {"width": 0, "slices": 0, "work": 0}
+    while len(out) < count:
         gseed = int(rng.integers(1 << 31))
-        if rng.random() < 0.5:
-            n = 2 * int(rng.integers(50, 131))  # n * 3 must be even
+        regular = rng.random() < 0.5
+        n = 2 * int(rng.integers(50, 201)) if regular else int(rng.integers(100, 251))
+        sc_target = int(rng.integers(15, 26))
+        f32 = rng.random() < 0.5
+        wseed = int(rng.integers(1 << 31))
+        if regular:
             g, kind = random_regular_graph(n, 3, seed=gseed), "3-regular"
         else:
-            n = int(rng.integers(100, 251))
             g, kind = erdos_renyi(n, 3.0 / (n - 1), seed=gseed), "ER"
-        lo = max(15, n // 10 - 3)
-        sc_target = int(rng.integers(lo, 26))
-        br = slice_graph(g, None, ContractionTreeSlicer(sc_target=sc_target,
-                                                        order_trials=order_trials), seed=gseed)
-        out.append((f"{kind} n={n} seed={gseed} sc_target={sc_target}", g, br))
+        dtype = np.float32 if f32 else np.int32
+        w = None if f32 else np.random.default_rng(wseed).integers(1, 101, g.nv).astype(np.int32)
+        width = complexity(SlicedBranch(MISProblem(g), initialize_code(g, 20, gseed), 0)).sc
+        if width - sc_target > max_width_gap:
+            counts["width"] += 1
+            continue
+        try:
+            br = slice_graph(g, w, ContractionTreeSlicer(sc_target=sc_target,
+                                                         order_trials=order_trials),
+                             seed=gseed, max_slices=max_slices)
+        except RuntimeError:
+            counts["slices"] += 1
+            continue
+        req = ServingRequest(kind, n, gseed, sc_target, dtype, g, w, br)
+        if max_log2_macs is not None and req.log2_macs > max_log2_macs:
+            counts["work"] += 1
+            continue
+        out.append(req)
+    if refused is not None:
+        refused.update(counts)
     return out

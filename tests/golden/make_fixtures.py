@@ -12,8 +12,9 @@ written only on request:
           sc_target 25: every per-slice optimum, written with the slice
           set's digest so that a test can regenerate the slices (the host
           slicer is deterministic) and check that they are the same ones;
-  --c5    the per-request optima of the seeded serving mix
-          (serving_requests(..., seed=5)) whose oracle work fits a budget.
+  --c5    the per-request optima of a seeded serving mix drawn from the
+          serving configuration's ranges (serving_requests(24, seed=5),
+          requests up to 2^36 MACs), each with its slice set's digest.
 """
 import hashlib
 import json
@@ -84,6 +85,38 @@ def write_c3(tb, oracle, nthreads=0):
     print(f"c3: {len(br)} slices, MIS {res.max()}, oracle {wall:.0f}s on {nthreads} threads")
 
 
+C5 = dict(count=24, seed=5, order_trials=50, max_slices=4096, max_log2_macs=36,
+          max_width_gap=16)
+
+
+def c5_requests(tb):
+    refused = {}
+    reqs = tb.serving_requests(C5["count"], seed=C5["seed"], order_trials=C5["order_trials"],
+                               max_slices=C5["max_slices"], max_log2_macs=C5["max_log2_macs"],
+                               max_width_gap=C5["max_width_gap"], refused=refused)
+    return reqs, refused
+
+
+def write_c5(tb, oracle):
+    """Per-request optima of the seeded serving mix, by the oracle; requests
+    with unit weights and n <= 120 are cross-checked against the exact
+    branch-and-reduce MIS."""
+    reqs, refused = c5_requests(tb)
+    out = []
+    for q in reqs:
+        res = oracle.contract_slices(q.branches, q.dtype, nthreads=os.cpu_count())
+        opt = res.max()
+        if q.weights is None and q.n <= 120:
+            assert int(opt) == oracle.mis_exact(q.graph), q.desc
+        out.append({"desc": q.desc, "dtype": "float32" if q.dtype == np.float32 else "int32",
+                    "slices": len(q.branches), "digest": sliceset_digest(q.branches),
+                    "log2_macs": round(q.log2_macs, 3),
+                    "optimum": float(opt) if q.dtype == np.float32 else int(opt)})
+    json.dump({"mix": tb.SERVING_MIX, "params": C5, "refused": refused, "requests": out},
+              open(os.path.join(HERE, "c5_serving_mix.json"), "w"), indent=0)
+    print(f"c5: {len(out)} requests, refused {refused}")
+
+
 def main():
     import __graft_entry__
     from oracle import oracle
@@ -143,9 +176,12 @@ def main():
 
 
 if __name__ == "__main__":
-    if "--c3" in sys.argv:
+    if "--c3" in sys.argv or "--c5" in sys.argv:
         import __graft_entry__
         from oracle import oracle
-        write_c3(__graft_entry__.package(), oracle)
+        if "--c3" in sys.argv:
+            write_c3(__graft_entry__.package(), oracle)
+        if "--c5" in sys.argv:
+            write_c5(__graft_entry__.package(), oracle)
     else:
         main()

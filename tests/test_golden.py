@@ -60,3 +60,41 @@ def test_gpu_reproduces_fixture(tb, backend, name, mode):
 def test_gpu_gemm_fixture(backend):
     d = json.load(open(os.path.join(HERE, "gemm_5x7x3.json")))
     assert np.array_equal(backend.maxplus_gemm(_dec(d["A"]), _dec(d["B"])), _dec(d["C"]))
+
+
+def _make_fixtures():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_fixtures",
+                                                  os.path.join(HERE, "make_fixtures.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_c3_fixture_slices_regenerate(tb, orc):
+    """The full-size fixture's slice set is what the host slicer produces
+    (same digest), and the oracle reproduces its least-work slices."""
+    mf = _make_fixtures()
+    d = json.load(open(os.path.join(HERE, "c3_n300_sc25.json")))
+    g, br = mf.c3_slices(tb)
+    assert len(br) == d["slices"] == 766
+    assert mf.sliceset_digest(br) == d["digest"]
+    tcs = np.array([np.inf if t is None else t for t in d["tc"]])
+    idx = np.argsort(tcs, kind="stable")[:12]
+    res = orc.contract_slices([br[k] for k in idx], np.float32)
+    assert res.tolist() == [d["per_slice"][k] for k in idx]
+    assert max(d["per_slice"]) == d["mis"]
+
+
+def test_c5_fixture_requests_regenerate(tb, orc):
+    """The serving-mix fixture's requests are what serving_requests draws
+    (same descriptions, refusals and digests), and the oracle reproduces the
+    stored optima of the smaller requests."""
+    mf = _make_fixtures()
+    d = json.load(open(os.path.join(HERE, "c5_serving_mix.json")))
+    reqs, refused = mf.c5_requests(tb)
+    assert refused == d["refused"]
+    for q, f in zip(reqs, d["requests"]):
+        assert q.desc == f["desc"] and mf.sliceset_digest(q.branches) == f["digest"]
+        if f["log2_macs"] <= 28:
+            assert orc.contract_slices(q.branches, q.dtype).max() == f["optimum"], q.desc

@@ -106,26 +106,48 @@ def test_cancel_queued(tb, be):
 
 
 def test_failed_request_does_not_block_others(tb, orc):
-    """A slice whose arena exceeds the pool fails its request; requests in
-    other rounds still complete."""
+    """A request with a slice whose arena exceeds the pool's region is
+    refused alone, also when it arrives in one burst with good requests
+    (no round cap, a 64 MiB pool): the good ones, before and after it in
+    the queue, finish with their oracle values."""
     be = tb.HIPBackend(0, pool_bytes=64 << 20)
     try:
         g_big = tb.random_regular_graph(200, 3, seed=8)  # unsliced: sc about 24
         code = tb.initialize_code(g_big, order_trials=10, seed=8)
         too_big = [tb.SlicedBranch(tb.MISProblem(g_big), code, 0)]
         assert tb.slice_footprint(too_big[0]) > (64 << 20)
-        g = tb.random_regular_graph(40, 3, seed=9)
-        ok = _sliced(tb, g, 6, seed=9)
+        good = []
+        for n, seed in [(40, 9), (60, 10), (50, 11)]:
+            g = tb.random_regular_graph(n, 3, seed=seed)
+            good.append((_sliced(tb, g, 6, seed=seed), orc.mis_exact(g)))
         with tb.Server(be) as srv:
-            t_ok0 = srv.submit(ok)
-            assert srv.wait(t_ok0) == orc.mis_exact(g)
+            t0 = srv.submit(good[0][0])
             t_bad = srv.submit(too_big)
-            with pytest.raises(tb.TbhError):
+            t1 = srv.submit(good[1][0])
+            t2 = srv.submit(good[2][0])
+            with pytest.raises(tb.TbhError, match="admission"):
                 srv.wait(t_bad)
-            t_ok = srv.submit(ok)
-            assert srv.wait(t_ok) == orc.mis_exact(g)
+            for t, (_, want) in zip((t0, t1, t2), good):
+                assert srv.wait(t) == want
+            st = srv.stats()
+            assert st["failed_requests"] == 1
+            # and a good request after the failure
+            assert srv.wait(srv.submit(good[1][0])) == good[1][1]
     finally:
         be.close()
+
+
+def test_round_work_bound_splits_requests(tb, orc, be):
+    """A small round work bound splits each request over many pipelined
+    rounds (a round always takes at least one slice); every value is
+    unchanged and equals the oracle's."""
+    reqs = _requests(tb, orc)
+    with tb.Server(be, round_log2_macs=12) as srv:
+        tickets = [srv.submit(br, dt) for br, dt, _ in reqs]
+        got = [srv.wait(t) for t in tickets]
+        rounds = srv.stats()["rounds"]
+    assert got == [want for _, _, want in reqs]
+    assert rounds > len(reqs)
 
 
 def test_submit_errors(tb, be):
@@ -141,26 +163,51 @@ def test_submit_errors(tb, be):
 
 
 def test_latency_lane(tb, orc, be):
-    """A priority request submitted behind a queue of large normal requests
-    is admitted into the next round: it finishes before most of them."""
-    big = []
+    """A latency-lane request (one slice, as slice_dfs_lp submits,
+    src/slice.jl:39) submitted while a large request is being served and a
+    queue of others waits behind it: it does not wait for that request or
+    the queue (rounds are bounded by work), it finishes before most of the
+    queued requests, well within the large request's own service time, and
+    its value equals the oracle's."""
+    g_big = tb.random_regular_graph(220, 3, seed=21)
+    big = tb.slice_graph(g_big, None, tb.ContractionTreeSlicer(sc_target=20, order_trials=30),
+                         seed=21)
+    queue = []
     for seed in range(6):
         g = tb.random_regular_graph(100, 3, seed=20 + seed)
-        big.append(_sliced(tb, g, 8, seed=seed))
+        queue.append(_sliced(tb, g, 8, seed=seed))
     g1 = tb.random_regular_graph(40, 3, seed=30)
     one = _sliced(tb, g1, 12, seed=30)[:1]
-    with tb.Server(be, max_resident_slices=8) as srv:
-        normal = [srv.submit(b) for b in big]
+    want = orc.contract_slices(one, np.float32)[0]
+    with tb.Server(be) as srv:
+        # the large request alone: its service time
+        t = srv.submit(big)
+        while True:
+            st, v_big, t0, t1 = srv.poll(t)
+            if st in ("done", "failed"):
+                break
+            time.sleep(0.001)
+        assert st == "done"
+        alone_s = t1 - t0
+        # again, with the queue behind it and the priority slice meanwhile
+        t_big = srv.submit(big)
+        normal = [srv.submit(b) for b in queue]
+        time.sleep(min(0.25 * alone_s, 0.05))
         fast = srv.submit(one, priority=1)
         done = {}
-        for t in normal + [fast]:
+        for tk in [t_big] + normal + [fast]:
             while True:
-                st, v, t0, t1 = srv.poll(t)
+                st, v, t0, t1 = srv.poll(tk)
                 if st in ("done", "failed"):
                     break
                 time.sleep(0.001)
             assert st == "done"
-            done[t] = t1
-    later = sum(1 for t in normal if done[t] > done[fast])
+            done[tk] = (v, t0, t1)
+    assert done[fast][0] == want
+    assert done[t_big][0] == v_big
+    later = sum(1 for tk in normal if done[tk][2] > done[fast][2])
     assert later >= len(normal) // 2, (later, done)
-    assert sum(len(b) for b in big) > 6 * 8  # the queue spans many rounds
+    lat_fast = done[fast][2] - done[fast][1]
+    print(f"big request alone {1e3 * alone_s:.1f} ms; priority slice {1e3 * lat_fast:.1f} ms")
+    if alone_s > 0.05:
+        assert lat_fast < 0.6 * alone_s, (lat_fast, alone_s)

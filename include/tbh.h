@@ -91,6 +91,22 @@ const char* tbh_last_error(void);
 int tbh_contract_slices(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
                         size_t n, void* results, tbh_mode mode);
 
+/* No counterpart in the reference: tbh_contract_slices(..., TBH_PLAIN) for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It allocates
+ * nothing: each slice's arena and step descriptors live in a static device
+ * array of TBH_NOALLOC_SCRATCH_BYTES (64 MiB) that every GPU the library runs
+ * on holds until the process ends; the context's pool is not touched.  A
+ * slice needing more than the array fails the call with TBH_ENOMEM before
+ * any work.  All calls on one GPU share its array, so the
+ * library runs them one at a time (a per-GPU lock; unserialized they would
+ * race on it); results equal tbh_contract_slices(..., TBH_PLAIN) bit for
+ * bit. */
+#define TBH_NOALLOC_SCRATCH_BYTES (64u << 20)
+int tbh_contract_slices_noalloc(tbh_ctx* ctx, tbh_dtype dtype,
+                                const tbh_slice* slices, size_t n,
+                                void* results);
+
 /* solve_slice(branch, T, usecuda) (src/dynamic_ob.jl:30-34): the contraction
  * of one slice WITHOUT r (the tropical one, 0, for a slice with no leaves). */
 int tbh_solve_slice(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slice,

@@ -107,6 +107,8 @@ TBH_SIGNATURES = {
     "tbh_serve_start_ex": (C.c_int, [C.c_void_p, C.POINTER(TbhServeConfig)]),
     "tbh_serve_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)]),
+    "tbh_contract_slices_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(TbhSlice),
+                                              C.c_size_t, C.c_void_p]),
     "tbh_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
 }
 

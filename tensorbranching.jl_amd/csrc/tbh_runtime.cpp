@@ -1063,6 +1063,38 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
 
 using namespace tbh;
 
+// ------------------------------------------- the no-allocation scratch ---
+// Device memory of tbh_contract_slices_noalloc: a file-scope array that the
+// code object holds on every GPU the library runs on, from load to process
+// end.  In host code `scratch` names a host-side stand-in, never the
+// device's copy: its device address is looked up once per device
+// (hipGetSymbolAddress, with that device current, in tbh_create) and only
+// that address is passed to kernels and copies.
+static __device__ __align__(256) char scratch[TBH_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;                        // guards the two tables
+char* g_scratch_ptr[kMaxDevices] = {};          // device address of `scratch`
+std::string g_scratch_err[kMaxDevices];         // why the lookup failed
+std::mutex g_scratch_use[kMaxDevices];          // one call per GPU at a time
+
+// Called from tbh_create with `device` current: resolves the device's copy of
+// the array once.
+void resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return;
+  }
+  g_scratch_ptr[device] = static_cast<char*>(p);
+}
+}  // namespace
+
 // ---------------------------------------------------------------- C ABI --
 extern "C" {
 
@@ -1119,6 +1151,7 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
   ctx->stats.pool_bytes = ctx->pool.capacity();
   ctx->plan_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   ctx->tune = default_tuning();
+  resolve_scratch(device);  // for tbh_contract_slices_noalloc (device is current)
   *out = ctx.release();
   return 0;
 }
@@ -1185,6 +1218,63 @@ static int contract_impl(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
 int tbh_contract_slices(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t n,
                         void* results, tbh_mode mode) {
   return contract_impl(ctx, dtype, slices, n, results, mode, true);
+}
+
+// tbh_contract_slices(..., TBH_PLAIN) with every device byte in the GPU's
+// copy of `scratch`: no hipMalloc / hipFree / pinned host allocation, no
+// pool.  Every slice is planned and measured before anything runs, so a
+// slice too large for the array fails the call before any work.  Nothing in
+// the array needs to start at zero: each slice's leaves, step outputs and
+// descriptors are written by this call before they are read.
+int tbh_contract_slices_noalloc(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t n,
+                                void* results) {
+  if (!ctx || (n && (!slices || !results))) {
+    set_error("null argument");
+    return TBH_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> gs(g_scratch_mu);
+    if (ctx->device < kMaxDevices) {
+      base = g_scratch_ptr[ctx->device];
+      if (!base) set_error(g_scratch_err[ctx->device]);
+    } else {
+      set_error("device index beyond the no-allocation table");
+    }
+  }
+  if (!base) return TBH_EDEVICE;
+  std::vector<SliceOwned> owned(n);
+  std::vector<const SliceOwned*> ptrs(n);
+  for (size_t i = 0; i < n; i++) {
+    int rc = convert_slice(slices[i], dtype, false, &owned[i]);
+    if (rc) {
+      set_error("slice " + std::to_string(i) + ": " + tbh_last_error());
+      return rc;
+    }
+    ptrs[i] = &owned[i];
+  }
+  std::vector<SlicePlan> plans;
+  int rc = plan_all(ctx, ptrs, false, &plans);
+  if (rc) return rc;
+  for (size_t i = 0; i < n; i++) {
+    const size_t need = plain_device_bytes(owned[i], plans[i]);
+    if (need > (size_t)TBH_NOALLOC_SCRATCH_BYTES) {
+      set_error("slice " + std::to_string(i) + " needs " + std::to_string(need) +
+                " bytes of device memory; tbh_contract_slices_noalloc holds at most " +
+                std::to_string((size_t)TBH_NOALLOC_SCRATCH_BYTES) +
+                " (64 MiB) per GPU: use tbh_contract_slices");
+      return TBH_ENOMEM;
+    }
+  }
+  std::lock_guard<std::mutex> gu(g_scratch_use[ctx->device]);
+  uint32_t* out = (uint32_t*)results;
+  for (size_t i = 0; i < n; i++) {
+    rc = execute_plain(ctx, dtype, owned[i], plans[i], true, out + i, base);
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 int tbh_solve_slice(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slice, void* result,

@@ -200,6 +200,38 @@ int tbh_wait(tbh_ctx* ctx, uint64_t ticket, void* result, tbh_status* status);
 /* Drops a queued (not yet admitted) request. */
 int tbh_cancel(tbh_ctx* ctx, uint64_t ticket);
 
+/* ---- branching tables of regions --------------------------------------- */
+/* A region of a MIS problem for the table step of optimal_branches
+ * (branching_table(p, TensorNetworkSolver(), region), src/branch.jl:79):
+ * its induced subgraph (vertex i = the region's i-th vertex), weights and
+ * boundary (open) vertices -- those with a neighbour outside the region.  A
+ * boundary configuration b has bit k for the region's k-th open vertex in
+ * ascending vertex order. */
+typedef struct {
+  int32_t nv;              /* region vertices (<= 24)                          */
+  const uint32_t* adj;     /* nv bitmasks: bit u of adj[v] set iff v ~ u       */
+  const void* weights;     /* nv dtype values (Float32: integers below 2^20;
+                            * Int32: |w| < 2^24); NULL = unit weights        */
+  uint32_t open;           /* bitmask of the open vertices (<= 16 of them)     */
+} tbh_region;
+/* alpha: per region, 2^popcount(open) values (regions concatenated): on
+ * entry the region's best weight with its open vertices fixed to each b (the
+ * tropical zero where b is not independent) -- the SizeMax contraction of
+ * the region's network with its boundary open, which callers obtain from
+ * tbh_contract_slices (one slice per b); on return the reduced alpha tensor
+ * (mis_compactify!: an entry is set to the tropical zero when a proper
+ * subset of its boundary configuration reaches at least its value).
+ * configs: every region configuration (bitmask over the region's vertices)
+ * that is independent and whose weight equals the reduced alpha of its
+ * boundary bits, i.e. the rows of the BranchingTable; region i's
+ * configurations are configs[group_ptr[i] .. group_ptr[i+1]), ordered by
+ * boundary configuration, then ascending.  With configs == NULL only
+ * group_ptr and *total (the number of configurations) are filled; a total
+ * above cap fails with TBH_ENOMEM (group_ptr and *total filled). */
+int tbh_region_tables(tbh_ctx* ctx, tbh_dtype dtype, const tbh_region* regions,
+                      size_t n, void* alpha, size_t* group_ptr,
+                      uint32_t* configs, size_t cap, size_t* total);
+
 /* ---- introspection ---------------------------------------------------- */
 /* One planned step of a slice (no reference counterpart: used by tests and
  * the bench to account algorithmic work per kernel). */

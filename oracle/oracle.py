@@ -198,3 +198,57 @@ def mwis_forest(g, weights=None):
     e = _edges(g)
     w = None if weights is None else np.ascontiguousarray(weights, np.float64)
     return lib().orc_mwis_forest(g.nv, g.ne(), e.ctypes.data, None if w is None else w.ctypes.data)
+
+
+def branching_table_bruteforce(g, weights, region):
+    """Branching table of a region by enumerating every configuration of its
+    vertices (bit i = region[i]): the open vertices (a neighbour outside the
+    region, in region order), the alpha tensor (per boundary configuration b,
+    bit k = k-th open vertex: the best weight of an independent set of the
+    region whose boundary part is b), the reduced tensor (an entry that a
+    proper subset of b matches or beats becomes -inf, the rule of
+    GenericTensorNetworks' mis_compactify! used by OptimalBranchingMIS's
+    reduced_alpha_configs for branching_table, src/branch.jl:79) and the
+    rows: per surviving b, ascending, the configurations reaching it.
+    Returns (open_vertices, alpha, reduced, rows) with -inf for zeros."""
+    region = [int(v) for v in region]
+    pos = {v: i for i, v in enumerate(region)}
+    nv = len(region)
+    adj = [0] * nv
+    nbrs = {}
+    for u, v in np.asarray(g.edges).reshape(-1, 2):
+        u, v = int(u), int(v)
+        nbrs.setdefault(u, set()).add(v)
+        nbrs.setdefault(v, set()).add(u)
+        if u in pos and v in pos:
+            adj[pos[u]] |= 1 << pos[v]
+            adj[pos[v]] |= 1 << pos[u]
+    ov = [v for v in region if any(x not in pos for x in nbrs.get(v, ()))]
+    opos = [pos[v] for v in ov]
+    w = np.ones(nv) if weights is None else np.asarray(weights, np.float64)[region]
+    x = np.arange(1 << nv, dtype=np.int64)
+    valid = np.ones(x.shape, bool)
+    wt = np.zeros(x.shape, np.float64)
+    for v in range(nv):
+        bit = (x >> v) & 1
+        valid &= ~((bit == 1) & ((x & adj[v]) != 0))
+        wt += bit * w[v]
+    b = np.zeros(x.shape, np.int64)
+    for k, p in enumerate(opos):
+        b |= ((x >> p) & 1) << k
+    nb = 1 << len(opos)
+    alpha = np.full(nb, -np.inf)
+    np.maximum.at(alpha, b[valid], wt[valid])
+    reduced = alpha.copy()
+    for a in range(nb):
+        for s in range(nb):
+            if s != a and (s & a) == s and alpha[s] >= alpha[a]:
+                reduced[a] = -np.inf
+                break
+    rows = []
+    for bb in range(nb):
+        if reduced[bb] == -np.inf:
+            continue
+        sel = valid & (b == bb) & (wt == reduced[bb])
+        rows.append(sorted(int(v) for v in x[sel]))
+    return ov, alpha, reduced, rows

@@ -19,6 +19,8 @@ from .workload import (  # noqa: F401
     ContractionTreeSlicer, dynamic_ob_mis, erdos_renyi, initialize_code, random_ksg,
     random_regular_graph, serving_requests, slice_graph, SERVING_MIX, ServingRequest)
 from .serve import Server  # noqa: F401
+from .branching import (  # noqa: F401
+    BranchingTable, branching_table, branching_tables, neighbor_region, open_vertices)
 from .io import (  # noqa: F401
     load_all_finished, load_code, load_finished, load_weights, loadgraph, save_code,
     save_finished, save_slices, save_weights, savegraph, threaded_saveslices)

@@ -29,6 +29,11 @@ class TbhServeConfig(C.Structure):
     _fields_ = [("max_resident_slices", C.c_size_t), ("round_log2_macs", C.c_double)]
 
 
+class TbhRegion(C.Structure):
+    _fields_ = [("nv", C.c_int32), ("adj", C.c_void_p), ("weights", C.c_void_p),
+                ("open", C.c_uint32)]
+
+
 class TbhRequest(C.Structure):
     _fields_ = [("dtype", C.c_int), ("slices", C.POINTER(TbhSlice)), ("n", C.c_size_t),
                 ("priority", C.c_int32)]
@@ -107,6 +112,9 @@ TBH_SIGNATURES = {
     "tbh_serve_start_ex": (C.c_int, [C.c_void_p, C.POINTER(TbhServeConfig)]),
     "tbh_serve_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)]),
+    "tbh_region_tables": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(TbhRegion), C.c_size_t,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                    C.POINTER(C.c_size_t)]),
     "tbh_contract_slices_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(TbhSlice),
                                               C.c_size_t, C.c_void_p]),
     "tbh_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),

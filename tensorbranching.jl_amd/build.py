@@ -21,7 +21,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-pthread",
               "-Wall", f"-I{INC}"]
 
-TBH_SOURCES = ["tbh_kernels.hip", "tbh_plan.cpp", "tbh_runtime.cpp", "tbh_serve.cpp"]
+TBH_SOURCES = ["tbh_kernels.hip", "tbh_plan.cpp", "tbh_runtime.cpp", "tbh_serve.cpp",
+               "tbh_regions.hip"]
 
 
 def _run(cmd):

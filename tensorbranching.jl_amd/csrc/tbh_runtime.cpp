@@ -286,6 +286,57 @@ PlanOptions plan_options(const tbh_ctx* ctx, bool batched) {
   return o;
 }
 
+void WorkerPool::start(int threads) {
+  stop();
+  stopping_ = false;
+  for (int t = 0; t < threads; t++)
+    th_.emplace_back([this] {
+      for (;;) {
+        Ticket tk;
+        {
+          std::unique_lock<std::mutex> g(m_);
+          cv_.wait(g, [&] { return stopping_ || !q_.empty(); });
+          if (q_.empty()) return;  // stopping, and nothing left to run
+          tk = std::move(q_.front());
+          q_.pop_front();
+        }
+        tk.b->body(tk.k);
+        {
+          std::lock_guard<std::mutex> g(tk.b->m);
+          tk.b->done++;
+        }
+        tk.b->cv.notify_all();
+      }
+    });
+}
+
+void WorkerPool::stop() {
+  {
+    std::lock_guard<std::mutex> g(m_);
+    stopping_ = true;
+  }
+  cv_.notify_all();
+  for (auto& t : th_) t.join();
+  th_.clear();
+}
+
+std::shared_ptr<WorkerPool::Batch> WorkerPool::post(int tickets, std::function<void(int)> body) {
+  auto b = std::make_shared<Batch>();
+  b->body = std::move(body);
+  b->total = tickets;
+  {
+    std::lock_guard<std::mutex> g(m_);
+    for (int k = 0; k < tickets; k++) q_.push_back({b, k});
+  }
+  cv_.notify_all();
+  return b;
+}
+
+void WorkerPool::wait(Batch& b) {
+  std::unique_lock<std::mutex> g(b.m);
+  b.cv.wait(g, [&] { return b.done >= b.total; });
+}
+
 int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool batched,
              std::vector<SlicePlan>* plans) {
   plans->assign(slices.size(), SlicePlan());
@@ -308,13 +359,12 @@ int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool ba
       }
     }
   };
-  int nt = (int)std::min<size_t>(std::max(1, ctx->plan_threads), slices.size());
+  const int nt = (int)std::min<size_t>(ctx->workers.size(), slices.size());
   if (nt <= 1) {
     worker();
   } else {
-    std::vector<std::thread> th;
-    for (int k = 0; k < nt; k++) th.emplace_back(worker);
-    for (auto& t : th) t.join();
+    auto b = ctx->workers.post(nt, [&](int) { worker(); });
+    WorkerPool::wait(*b);
   }
   if (bad.load() >= 0) {
     set_error(first_err);
@@ -324,9 +374,9 @@ int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool ba
 }
 
 PlanStream::PlanStream(const std::vector<const SliceOwned*>& slices, const PlanOptions& opt,
-                       int threads)
+                       WorkerPool& pool)
     : slices_(slices), plans_(slices.size()), done_(slices.size(), 0) {
-  auto worker = [this, opt]() {
+  auto worker = [this, opt](int) {
     for (;;) {
       const size_t i = next_.fetch_add(1);
       if (i >= slices_.size()) return;
@@ -347,8 +397,12 @@ PlanStream::PlanStream(const std::vector<const SliceOwned*>& slices, const PlanO
       cv_.notify_all();
     }
   };
-  const int nt = (int)std::min<size_t>(std::max(1, threads), std::max<size_t>(1, slices.size()));
-  for (int k = 0; k < nt; k++) th_.emplace_back(worker);
+  const int nt = (int)std::min<size_t>(std::max(1, pool.size()), std::max<size_t>(1, slices.size()));
+  if (pool.size() == 0) {
+    worker(0);  // no pool threads: plan everything now
+  } else {
+    batch_ = pool.post(nt, worker);
+  }
 }
 
 PlanStream::~PlanStream() {
@@ -356,7 +410,7 @@ PlanStream::~PlanStream() {
     std::lock_guard<std::mutex> g(m_);
     bad_ = 0;  // stop the workers after their current slice
   }
-  for (auto& t : th_) t.join();
+  if (batch_) WorkerPool::wait(*batch_);
 }
 
 const SlicePlan* PlanStream::get(size_t i, std::string* err) {
@@ -473,21 +527,20 @@ int64_t op_tiles(const POp& op) {
 size_t desc_size(int kind) { return kind >= 2 ? sizeof(GemmDesc) : sizeof(EwDesc); }
 int stat_kind(int kind) { return kind < 2 ? kind : 2; }  // tbh_stats: direct, tiled, GEMM
 
-// Runs f(i) for i in [0, n) on up to `threads` host threads.
+// Runs f(i, t) for i in [0, n) on up to the pool's threads (t < pool size:
+// the running thread's slot), or on the caller for small n.
 template <typename F>
-void parallel_for(size_t n, int threads, F f) {
-  const int nt = (int)std::min<size_t>(std::max(1, threads), std::max<size_t>(1, n / 16));
+void parallel_for(size_t n, WorkerPool& pool, F f) {
+  const int nt = (int)std::min<size_t>(pool.size(), std::max<size_t>(1, n / 16));
   if (nt <= 1) {
     for (size_t i = 0; i < n; i++) f(i, 0);
     return;
   }
   std::atomic<size_t> next{0};
-  std::vector<std::thread> pool;
-  for (int t = 0; t < nt; t++)
-    pool.emplace_back([&, t]() {
-      for (size_t i; (i = next.fetch_add(1)) < n;) f(i, t);
-    });
-  for (auto& th : pool) th.join();
+  auto b = pool.post(nt, [&](int t) {
+    for (size_t i; (i = next.fetch_add(1)) < n;) f(i, t);
+  });
+  WorkerPool::wait(*b);
 }
 
 }  // namespace
@@ -673,7 +726,7 @@ int BatchRunner::enqueue(int dtype, const std::vector<const SliceOwned*>& slices
   // pass 2
   std::vector<std::vector<int32_t>> local(std::max(1, ctx->plan_threads),
                                           std::vector<int32_t>(W));
-  parallel_for(nb, ctx->plan_threads, [&](size_t i, int t) {
+  parallel_for(nb, ctx->workers, [&](size_t i, int t) {
     const SlicePlan& P = *plans[i];
     const uint64_t b = sbase[i];
     EpiDesc e;
@@ -785,7 +838,6 @@ int BatchRunner::enqueue(int dtype, const std::vector<const SliceOwned*>& slices
             std::snprintf(tag_s, sizeof tag_s, "M=%d N=%d K=%d batch=%d", d.M, d.N, d.K, d.batch);
           } else {
             const EwDesc& d = reinterpret_cast<const EwDesc*>(hb + o_desc[w])[q];
-            if (k == 0 && d.nO < 8) continue;  // only the big direct steps
             std::snprintf(tag_s, sizeof tag_s, "nT=%d nO=%d nR=%d nRo=%d nSA=%d nSB=%d op=%d",
                           d.nT, d.nO, d.nR, d.nRo, d.nSA, d.nSB, d.mlay);
           }
@@ -1150,6 +1202,7 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
   ctx->pool.init(pool_bytes);
   ctx->stats.pool_bytes = ctx->pool.capacity();
   ctx->plan_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  ctx->workers.start(ctx->plan_threads);
   ctx->tune = default_tuning();
   resolve_scratch(device);  // for tbh_contract_slices_noalloc (device is current)
   *out = ctx.release();
@@ -1199,7 +1252,7 @@ static int contract_impl(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
   uint32_t* out = (uint32_t*)results;
   if (mode == TBH_BATCHED) {
     // slices are planned on host threads while earlier batches run
-    PlanStream ps(ptrs, plan_options(ctx, true), ctx->plan_threads);
+    PlanStream ps(ptrs, plan_options(ctx, true), ctx->workers);
     ctx->stats.host_ms[0] += ms_since(t_plan);
     return execute_batched(ctx, dtype, ptrs,
                            [&](size_t i, std::string* err) { return ps.get(i, err); }, add_r, out);

@@ -64,6 +64,40 @@ struct Staging {  // pinned host buffer reused for descriptor uploads
 
 struct ServeState;
 
+// Host threads that live as long as the context: slice planning, descriptor
+// packing and the serving planner post work to them instead of starting
+// threads per call (thread start-up was a third of a small solve's host
+// time).  post() queues `tickets` runs of body(k), k = 0..tickets-1, and
+// returns at once; wait() blocks until all of them have returned.  Bodies
+// never wait for other posted work, so posts from several threads cannot
+// deadlock.
+class WorkerPool {
+ public:
+  struct Batch {
+    std::function<void(int)> body;
+    int total = 0, done = 0;
+    std::mutex m;
+    std::condition_variable cv;
+  };
+  void start(int threads);
+  void stop();
+  ~WorkerPool() { stop(); }
+  int size() const { return (int)th_.size(); }
+  std::shared_ptr<Batch> post(int tickets, std::function<void(int)> body);
+  static void wait(Batch& b);
+
+ private:
+  struct Ticket {
+    std::shared_ptr<Batch> b;
+    int k;
+  };
+  std::mutex m_;
+  std::condition_variable cv_;
+  std::deque<Ticket> q_;
+  bool stopping_ = false;
+  std::vector<std::thread> th_;
+};
+
 // HIP-event timing of launches on the context's stream (profiling mode).
 struct Profiler {
   struct Rec {
@@ -103,6 +137,7 @@ struct tbh_ctx {
   std::mutex mu;         // one executor call at a time
   tbh_stats stats{};
   int plan_threads = 8;
+  tbh::WorkerPool workers;  // plan_threads host threads (planning, packing)
   tbh_tuning tune{};   // planner / launch parameters (tbh_set_tuning, tbh_autotune)
   bool profiling = false;
   tbh::Profiler prof;
@@ -128,7 +163,8 @@ int convert_slice(const tbh_slice& s, tbh_dtype dtype, bool copy, SliceOwned* ou
 // failed at or before i.
 class PlanStream {
  public:
-  PlanStream(const std::vector<const SliceOwned*>& slices, const PlanOptions& opt, int threads);
+  PlanStream(const std::vector<const SliceOwned*>& slices, const PlanOptions& opt,
+             WorkerPool& pool);
   ~PlanStream();
   const SlicePlan* get(size_t i, std::string* err);
 
@@ -141,7 +177,7 @@ class PlanStream {
   std::atomic<size_t> next_{0};
   size_t bad_ = (size_t)-1;
   std::string bad_err_;
-  std::vector<std::thread> th_;
+  std::shared_ptr<WorkerPool::Batch> batch_;
 };
 
 // The arena region a batched run takes when `largest` bytes are the

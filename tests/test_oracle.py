@@ -211,3 +211,38 @@ def test_peak_and_all_memory_hand_computed(tb, orc):
     assert sc == 1
     # rw: (2+4+2) + (2+4+2) + (2+2+2) + (2+2+1)
     assert rwc == pytest.approx(np.log2(27))
+
+
+def test_branching_table_oracle_hand_worked(tb, orc):
+    """The brute-force table oracle on tables worked out by hand.
+    Path 0-1-2, region [0, 1]: vertex 1 is open (neighbour 2 outside).
+    Unit weights: alpha = [1 ({0}), 1 ({1})]; taking 1 is no better than not
+    (the empty boundary configuration reaches 1 too), so only b = 0 stays,
+    with the one configuration {0} = 0b01."""
+    g = tb.SimpleGraph(3, np.array([[0, 1], [1, 2]], np.int32))
+    ov, alpha, red, rows = orc.branching_table_bruteforce(g, None, [0, 1])
+    assert ov == [1]
+    assert alpha.tolist() == [1.0, 1.0] and red.tolist() == [1.0, -np.inf]
+    assert rows == [[0b01]]
+    # weight 5 on the open vertex: taking it is better, both rows stay
+    ov, alpha, red, rows = orc.branching_table_bruteforce(g, [1.0, 5.0, 1.0], [0, 1])
+    assert alpha.tolist() == [1.0, 5.0] and red.tolist() == [1.0, 5.0]
+    assert rows == [[0b01], [0b10]]
+    # star: centre 0 (open, neighbour 4 outside) with leaves 1, 2, 3 inside:
+    # b = 0 reaches 3 with the three leaves, b = 1 only 1 -> pruned
+    g = tb.SimpleGraph(5, np.array([[0, 1], [0, 2], [0, 3], [0, 4]], np.int32))
+    ov, alpha, red, rows = orc.branching_table_bruteforce(g, None, [0, 1, 2, 3])
+    assert ov == [0] and alpha.tolist() == [3.0, 1.0] and red.tolist() == [3.0, -np.inf]
+    assert rows == [[0b1110]]
+    # two open ends of a path 0-1-2-3 inside a longer path (-1 and 4 are
+    # outside): region [1, 2, 3] with 1 and 3 open; configurations:
+    # b=00: {2} -> 1; b=01 (1 taken): {1, 3}? 3 is open and not taken, so
+    # {1} -> 1, pruned by b=00; b=10 (3 taken): {3} -> 1, pruned; b=11:
+    # {1, 3} -> 2, kept (its proper subsets reach only 1); rows: [{2}],
+    # [{1, 3}] = [0b010], [0b101]
+    g = tb.SimpleGraph(5, np.array([[0, 1], [1, 2], [2, 3], [3, 4]], np.int32))
+    ov, alpha, red, rows = orc.branching_table_bruteforce(g, None, [1, 2, 3])
+    assert ov == [1, 3]
+    assert alpha.tolist() == [1.0, 1.0, 1.0, 2.0]
+    assert red.tolist() == [1.0, -np.inf, -np.inf, 2.0]
+    assert rows == [[0b010], [0b101]]

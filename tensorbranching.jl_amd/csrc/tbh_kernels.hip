@@ -144,8 +144,10 @@ __device__ __forceinline__ uint32_t hi_at(const uint32_t (&h)[4], int m) {
 __device__ __forceinline__ uint32_t lo_dep(uint32_t tid, KI8 pos) {
   uint32_t v = 0;
 #pragma unroll
-  for (int b = 0; b < 8; b++)
-    if ((tid >> b) & 1) v += pos_stride(pos[b]);
+  for (int b = 0; b < 8; b++) {
+    const uint32_t st = pos_stride(pos[b]);  // the same in every lane: read it in all
+    v += ((tid >> b) & 1) ? st : 0u;
+  }
   return v;
 }
 

@@ -300,10 +300,19 @@ void WorkerPool::start(int threads) {
           tk = std::move(q_.front());
           q_.pop_front();
         }
+        {
+          std::lock_guard<std::mutex> g(tk.b->m);
+          if (tk.b->closed) {  // its poster finished the work itself
+            tk.b->skipped++;
+            tk.b->cv.notify_all();
+            continue;
+          }
+          tk.b->started++;
+        }
         tk.b->body(tk.k);
         {
           std::lock_guard<std::mutex> g(tk.b->m);
-          tk.b->done++;
+          tk.b->finished++;
         }
         tk.b->cv.notify_all();
       }
@@ -334,7 +343,13 @@ std::shared_ptr<WorkerPool::Batch> WorkerPool::post(int tickets, std::function<v
 
 void WorkerPool::wait(Batch& b) {
   std::unique_lock<std::mutex> g(b.m);
-  b.cv.wait(g, [&] { return b.done >= b.total; });
+  b.cv.wait(g, [&] { return b.finished + b.skipped >= b.total; });
+}
+
+void WorkerPool::close_and_wait(Batch& b) {
+  std::unique_lock<std::mutex> g(b.m);
+  b.closed = true;  // tickets that have not started skip the body from now on
+  b.cv.wait(g, [&] { return b.finished >= b.started; });
 }
 
 int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool batched,
@@ -359,12 +374,14 @@ int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool ba
       }
     }
   };
-  const int nt = (int)std::min<size_t>(ctx->workers.size(), slices.size());
+  const int nt = (int)std::min<size_t>(ctx->plan_threads, slices.size());
   if (nt <= 1) {
     worker();
   } else {
-    auto b = ctx->workers.post(nt, [&](int) { worker(); });
-    WorkerPool::wait(*b);
+    // nt - 1 pool threads and this one
+    auto b = ctx->workers.post(nt - 1, [&](int) { worker(); });
+    worker();
+    WorkerPool::close_and_wait(*b);
   }
   if (bad.load() >= 0) {
     set_error(first_err);
@@ -374,7 +391,7 @@ int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool ba
 }
 
 PlanStream::PlanStream(const std::vector<const SliceOwned*>& slices, const PlanOptions& opt,
-                       WorkerPool& pool)
+                       WorkerPool& pool, int threads)
     : slices_(slices), plans_(slices.size()), done_(slices.size(), 0) {
   auto worker = [this, opt](int) {
     for (;;) {
@@ -397,7 +414,7 @@ PlanStream::PlanStream(const std::vector<const SliceOwned*>& slices, const PlanO
       cv_.notify_all();
     }
   };
-  const int nt = (int)std::min<size_t>(std::max(1, pool.size()), std::max<size_t>(1, slices.size()));
+  const int nt = (int)std::min<size_t>(std::max(1, threads), std::max<size_t>(1, slices.size()));
   if (pool.size() == 0) {
     worker(0);  // no pool threads: plan everything now
   } else {
@@ -527,20 +544,22 @@ int64_t op_tiles(const POp& op) {
 size_t desc_size(int kind) { return kind >= 2 ? sizeof(GemmDesc) : sizeof(EwDesc); }
 int stat_kind(int kind) { return kind < 2 ? kind : 2; }  // tbh_stats: direct, tiled, GEMM
 
-// Runs f(i, t) for i in [0, n) on up to the pool's threads (t < pool size:
-// the running thread's slot), or on the caller for small n.
+// Runs f(i, t) for i in [0, n) on the caller (slot 0) and up to threads - 1
+// pool threads (slots 1..), or on the caller alone for small n; pool threads
+// busy elsewhere never hold it up.
 template <typename F>
-void parallel_for(size_t n, WorkerPool& pool, F f) {
-  const int nt = (int)std::min<size_t>(pool.size(), std::max<size_t>(1, n / 16));
-  if (nt <= 1) {
+void parallel_for(size_t n, WorkerPool& pool, int threads, F f) {
+  const int nt = (int)std::min<size_t>(std::max(1, threads), std::max<size_t>(1, n / 16));
+  if (nt <= 1 || pool.size() == 0) {
     for (size_t i = 0; i < n; i++) f(i, 0);
     return;
   }
   std::atomic<size_t> next{0};
-  auto b = pool.post(nt, [&](int t) {
-    for (size_t i; (i = next.fetch_add(1)) < n;) f(i, t);
+  auto b = pool.post(nt - 1, [&](int t) {
+    for (size_t i; (i = next.fetch_add(1)) < n;) f(i, t + 1);
   });
-  WorkerPool::wait(*b);
+  for (size_t i; (i = next.fetch_add(1)) < n;) f(i, 0);
+  WorkerPool::close_and_wait(*b);
 }
 
 }  // namespace
@@ -726,7 +745,7 @@ int BatchRunner::enqueue(int dtype, const std::vector<const SliceOwned*>& slices
   // pass 2
   std::vector<std::vector<int32_t>> local(std::max(1, ctx->plan_threads),
                                           std::vector<int32_t>(W));
-  parallel_for(nb, ctx->workers, [&](size_t i, int t) {
+  parallel_for(nb, ctx->workers, ctx->plan_threads, [&](size_t i, int t) {
     const SlicePlan& P = *plans[i];
     const uint64_t b = sbase[i];
     EpiDesc e;
@@ -1202,7 +1221,7 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
   ctx->pool.init(pool_bytes);
   ctx->stats.pool_bytes = ctx->pool.capacity();
   ctx->plan_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  ctx->workers.start(ctx->plan_threads);
+  ctx->workers.start(2 * ctx->plan_threads);
   ctx->tune = default_tuning();
   resolve_scratch(device);  // for tbh_contract_slices_noalloc (device is current)
   *out = ctx.release();
@@ -1252,7 +1271,7 @@ static int contract_impl(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices,
   uint32_t* out = (uint32_t*)results;
   if (mode == TBH_BATCHED) {
     // slices are planned on host threads while earlier batches run
-    PlanStream ps(ptrs, plan_options(ctx, true), ctx->workers);
+    PlanStream ps(ptrs, plan_options(ctx, true), ctx->workers, ctx->plan_threads);
     ctx->stats.host_ms[0] += ms_since(t_plan);
     return execute_batched(ctx, dtype, ptrs,
                            [&](size_t i, std::string* err) { return ps.get(i, err); }, add_r, out);

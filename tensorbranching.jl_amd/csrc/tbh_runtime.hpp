@@ -66,16 +66,20 @@ struct ServeState;
 
 // Host threads that live as long as the context: slice planning, descriptor
 // packing and the serving planner post work to them instead of starting
-// threads per call (thread start-up was a third of a small solve's host
-// time).  post() queues `tickets` runs of body(k), k = 0..tickets-1, and
-// returns at once; wait() blocks until all of them have returned.  Bodies
+// threads per call.  The pool holds two threads per planning thread, so a
+// batch can be packed while later slices are still being planned.  post()
+// queues `tickets` runs of body(k), k = 0..tickets-1, and returns at once;
+// wait() blocks until all of them have returned.  Bodies
 // never wait for other posted work, so posts from several threads cannot
 // deadlock.
 class WorkerPool {
  public:
   struct Batch {
     std::function<void(int)> body;
-    int total = 0, done = 0;
+    int total = 0;
+    int started = 0, finished = 0;  // tickets that ran the body
+    int skipped = 0;                // tickets that found the batch closed
+    bool closed = false;            // tickets not started yet skip the body
     std::mutex m;
     std::condition_variable cv;
   };
@@ -85,6 +89,9 @@ class WorkerPool {
   int size() const { return (int)th_.size(); }
   std::shared_ptr<Batch> post(int tickets, std::function<void(int)> body);
   static void wait(Batch& b);
+  // closes b (its tickets not started yet will not run) and waits for the
+  // ones running: for posters that do the work themselves as well
+  static void close_and_wait(Batch& b);
 
  private:
   struct Ticket {
@@ -164,7 +171,7 @@ int convert_slice(const tbh_slice& s, tbh_dtype dtype, bool copy, SliceOwned* ou
 class PlanStream {
  public:
   PlanStream(const std::vector<const SliceOwned*>& slices, const PlanOptions& opt,
-             WorkerPool& pool);
+             WorkerPool& pool, int threads);
   ~PlanStream();
   const SlicePlan* get(size_t i, std::string* err);
 

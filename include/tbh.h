@@ -269,8 +269,13 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode,
  * direct_max_reduce: EW steps reducing more labels than this run LDS-tiled,
  * not as direct wave tiles (-1: all tiled); concurrent: 1 runs a level's
  * GEMM launches on a second stream beside its EW launches; first_batch:
- * fraction of the arena region the first batch may use.  Results never
- * depend on them. */
+ * fraction of the arena region the first batch may use; ew_prefetch: 1 runs
+ * LDS-tiled EW steps with the next tile's sub-tensors loaded into registers
+ * while the current tile computes, 0 stages each tile by DMA after the
+ * previous one; ew_grid_mult / gemm_grid_mult: a tiled EW / GEMM launch's
+ * grid holds this many times the workgroups that stay resident (1: a
+ * persistent grid; more: later workgroups start as earlier ones finish,
+ * which evens out uneven tiles).  Results never depend on them. */
 typedef struct {
   int32_t chain_big, chain_side, chain_reduce;
   int32_t split_tiles_bits, split_min_k_bits;
@@ -279,6 +284,8 @@ typedef struct {
   int32_t direct_max_reduce;
   int32_t concurrent;
   double first_batch;
+  int32_t ew_prefetch;
+  int32_t ew_grid_mult, gemm_grid_mult;
 } tbh_tuning;
 int tbh_get_tuning(tbh_ctx* ctx, tbh_tuning* out);
 int tbh_set_tuning(tbh_ctx* ctx, const tbh_tuning* t);

@@ -57,7 +57,29 @@ struct KI8 {
     return (int)(int8_t)(w >> ((o & 3) * 8));
   }
 };
-#define TBH_KF(d, f) (KI8{(d), (uint32_t)offsetof(EwDesc, f)})
+// The same through the constant address space: scalar loads, counted by
+// lgkmcnt only (the prefetching EW kernel keeps vmcnt for its staging loads).
+using KEwS = const __attribute__((address_space(4))) EwDesc*;
+struct KI8S {
+  KEwS d;
+  uint32_t off;
+  __device__ __forceinline__ int operator[](uint32_t i) const {
+    const uint32_t o = off + i;
+    const uint32_t w = reinterpret_cast<const __attribute__((address_space(4))) uint32_t*>(d)[o >> 2];
+    return (int)(int8_t)(w >> ((o & 3) * 8));
+  }
+};
+template <class D>
+struct KAcc;
+template <>
+struct KAcc<KEw> {
+  using type = KI8;
+};
+template <>
+struct KAcc<KEwS> {
+  using type = KI8S;
+};
+#define TBH_KF(d, f) (typename KAcc<decltype(d)>::type{(d), (uint32_t)offsetof(EwDesc, f)})
 // Tensors are addressed in the global address space: their loads are
 // global_load (counted by vmcnt only), not flat loads, which also count in
 // lgkmcnt and so would make every wait for an LDS or scalar access wait for
@@ -126,7 +148,8 @@ __device__ __forceinline__ int32_t pad_zero<int32_t>() { return kI32NegInf; }
 // ---------------------------------------------------------------- K2 -----
 // The 4 high bits (8..11) of a per-thread index: strides of those bits, and
 // the sum over the bits set in a compile-time m.
-__device__ __forceinline__ void hi_strides(uint32_t (&h)[4], KI8 pos) {
+template <class K>
+__device__ __forceinline__ void hi_strides(uint32_t (&h)[4], K pos) {
 #pragma unroll
   for (int b = 0; b < 4; b++) h[b] = pos_stride(pos[8 + b]);
 }
@@ -141,7 +164,8 @@ __device__ __forceinline__ uint32_t hi_at(const uint32_t (&h)[4], int m) {
 }
 
 // the low 8 bits of the thread index mapped through pos[0..7]
-__device__ __forceinline__ uint32_t lo_dep(uint32_t tid, KI8 pos) {
+template <class K>
+__device__ __forceinline__ uint32_t lo_dep(uint32_t tid, K pos) {
   uint32_t v = 0;
 #pragma unroll
   for (int b = 0; b < 8; b++) {
@@ -160,7 +184,8 @@ struct EwThread {  // per-step constants of one thread
   bool x4A, x4B;
 };
 
-__device__ __forceinline__ void ew_thread_setup(KEw d, EwThread& e) {
+template <class D>
+__device__ __forceinline__ void ew_thread_setup(D d, EwThread& e) {
   const uint32_t tid = threadIdx.x;
   e.loSA = lo_dep(tid, TBH_KF(d, sA));
   e.loSB = lo_dep(tid, TBH_KF(d, sB));
@@ -172,12 +197,13 @@ __device__ __forceinline__ void ew_thread_setup(KEw d, EwThread& e) {
   hi_strides(e.hTA, TBH_KF(d, tA));
   hi_strides(e.hTB, TBH_KF(d, tB));
   hi_strides(e.hTC, TBH_KF(d, tC));
-  const KI8 sA = TBH_KF(d, sA), sB = TBH_KF(d, sB);
+  using K = typename KAcc<D>::type;
+  const K sA = TBH_KF(d, sA), sB = TBH_KF(d, sB);
   constexpr bool x4 = TBH_LDS_X4 && TBH_LDS_DMA && TBH_LDS_PAD_SHIFT == 0;
   e.x4A = x4 && d->nSA >= 10 && sA[0] == 0 && sA[1] == 1;
   e.x4B = x4 && d->nSB >= 10 && sB[0] == 0 && sB[1] == 1;
-  e.lo4SA = lo_dep(tid, KI8{d, sA.off + 2});
-  e.lo4SB = lo_dep(tid, KI8{d, sB.off + 2});
+  e.lo4SA = lo_dep(tid, K{d, sA.off + 2});
+  e.lo4SB = lo_dep(tid, K{d, sB.off + 2});
 }
 
 // The step index and its tile range are the same in every lane of a wave;
@@ -216,7 +242,8 @@ __device__ __forceinline__ int find_step(const int64_t* __restrict__ prefix, int
 struct EwOuter {
   uint32_t a, b, c;
 };
-__device__ __forceinline__ EwOuter outer_offsets(KEw d, uint32_t o) {
+template <class D>
+__device__ __forceinline__ EwOuter outer_offsets(D d, uint32_t o) {
   EwOuter r{0, 0, 0};
   const int nO = d->nO;
 #pragma unroll
@@ -229,7 +256,8 @@ __device__ __forceinline__ EwOuter outer_offsets(KEw d, uint32_t o) {
   return r;
 }
 // from tile o-1 to tile o: only the bits that flip change
-__device__ __forceinline__ void outer_step(KEw d, uint32_t o, EwOuter& r) {
+template <class D>
+__device__ __forceinline__ void outer_step(D d, uint32_t o, EwOuter& r) {
   uint32_t flip = o ^ (o - 1);
   while (flip) {
     const int k = __builtin_ctz(flip);
@@ -313,6 +341,137 @@ __device__ __forceinline__ void ew_stage(const TBH_GLOBAL T* __restrict__ X, int
 #endif
 }
 
+// Max A + B over the sub-tensor reduced configurations of the staged unit
+// into acc (one thread's CNT outputs).  The lowest (up to 3) sub-tensor
+// reduced bits are unrolled in pairs (add + max3), the rest walked in
+// Gray-code order.  D is the descriptor pointer (generic or constant address
+// space: the prefetching kernel reads descriptors with scalar loads only).
+template <typename T, bool HASB, int CNT, int MA, class D>
+__device__ __forceinline__ void ew_compute(D d, const EwThread& e, const T* __restrict__ lA,
+                                           const T* __restrict__ lB, T (&acc)[CNT]) {
+  const int nR = d->nR;
+  const int nlo = nR < 3 ? nR : 3;
+  uint32_t lsA[4] = {0, 0, 0, 0}, lsB[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int b = 0; b < 3; b++)
+    if (b < nlo) {
+      lsA[b] = pos_stride(TBH_KF(d, rA)[b]);
+      lsB[b] = pos_stride(TBH_KF(d, rB)[b]);
+    }
+  const int nq = 1 << nlo;
+  const uint32_t nh = 1u << (nR - nlo);
+  if (MA > 0) {
+    constexpr int NA = 1 << MA, NB = 16 >> MA;
+    uint32_t ltA = e.loTA, ltB = e.loTB;
+    asm volatile("" : "+v"(ltA), "+v"(ltB));
+    uint32_t pA[NA], pB[NB];
+#pragma unroll
+    for (int j = 0; j < NA; j++) pA[j] = lds_pad(ltA + hi_at(e.hTA, j));
+#pragma unroll
+    for (int j = 0; j < NB; j++) pB[j] = lds_pad(ltB + hi_at(e.hTB, j << MA));
+    uint32_t rA = 0, rB = 0;
+    for (uint32_t h = 0; h < nh; h++) {
+      if (h) {
+        const int i = __builtin_ctz(h);
+        const uint32_t g = h ^ (h >> 1);
+        const uint32_t sa = pos_stride(TBH_KF(d, rA)[nlo + i]), sb = pos_stride(TBH_KF(d, rB)[nlo + i]);
+        if ((g >> i) & 1) {
+          rA += sa;
+          rB += sb;
+        } else {
+          rA -= sa;
+          rB -= sb;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q += 2) {
+        if (q >= nq) break;
+        const uint32_t qa0 = lds_pad(rA + hi_at(lsA, q)), qb0 = lds_pad(rB + hi_at(lsB, q));
+        T a0[NA], b0[NB];
+#pragma unroll
+        for (int j = 0; j < NA; j++) a0[j] = lA[pA[j] + qa0];
+#pragma unroll
+        for (int j = 0; j < NB; j++) b0[j] = lB[pB[j] + qb0];
+        if (nq > 1) {
+          const uint32_t qa1 = lds_pad(rA + hi_at(lsA, q + 1));
+          const uint32_t qb1 = lds_pad(rB + hi_at(lsB, q + 1));
+          T a1[NA], b1[NB];
+#pragma unroll
+          for (int j = 0; j < NA; j++) a1[j] = lA[pA[j] + qa1];
+#pragma unroll
+          for (int j = 0; j < NB; j++) b1[j] = lB[pB[j] + qb1];
+#pragma unroll
+          for (int mb = 0; mb < NB; mb++)
+#pragma unroll
+            for (int ma = 0; ma < NA; ma++)
+              acc[ma + NA * mb] = max3(acc[ma + NA * mb], Trop<T>::mul(a0[ma], b0[mb]),
+                                       Trop<T>::mul(a1[ma], b1[mb]));
+        } else {
+#pragma unroll
+          for (int mb = 0; mb < NB; mb++)
+#pragma unroll
+            for (int ma = 0; ma < NA; ma++)
+              acc[ma + NA * mb] = Trop<T>::add(acc[ma + NA * mb], Trop<T>::mul(a0[ma], b0[mb]));
+        }
+      }
+    }
+    return;
+  }
+  {
+    // per-thread LDS addresses, recomputed per chunk: the empty asm hides
+    // that they are loop-invariant, so they are not held in registers
+    // across the staging
+    uint32_t ltA = e.loTA, ltB = e.loTB;
+    asm volatile("" : "+v"(ltA), "+v"(ltB));
+    uint32_t pA[CNT], pB[CNT];
+#pragma unroll
+    for (int m = 0; m < CNT; m++) {
+      pA[m] = lds_pad(ltA + hi_at(e.hTA, m));
+      if (HASB) pB[m] = lds_pad(ltB + hi_at(e.hTB, m));
+    }
+    uint32_t rA = 0, rB = 0;
+    for (uint32_t h = 0; h < nh; h++) {
+      if (h) {
+        const int i = __builtin_ctz(h);
+        const uint32_t g = h ^ (h >> 1);
+        const uint32_t sa = pos_stride(TBH_KF(d, rA)[nlo + i]), sb = pos_stride(TBH_KF(d, rB)[nlo + i]);
+        if ((g >> i) & 1) {
+          rA += sa;
+          rB += sb;
+        } else {
+          rA -= sa;
+          rB -= sb;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q += 2) {
+        if (q >= nq) break;
+        const uint32_t qa0 = lds_pad(rA + hi_at(lsA, q)), qb0 = lds_pad(rB + hi_at(lsB, q));
+        if (nq > 1) {
+          const uint32_t qa1 = lds_pad(rA + hi_at(lsA, q + 1));
+          const uint32_t qb1 = lds_pad(rB + hi_at(lsB, q + 1));
+#pragma unroll
+          for (int m = 0; m < CNT; m++) {
+            T x0 = lA[pA[m] + qa0], x1 = lA[pA[m] + qa1];
+            if (HASB) {
+              x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
+              x1 = Trop<T>::mul(x1, lB[pB[m] + qb1]);
+            }
+            acc[m] = max3(acc[m], x0, x1);
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < CNT; m++) {
+            T x0 = lA[pA[m] + qa0];
+            if (HASB) x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
+            acc[m] = Trop<T>::add(acc[m], x0);
+          }
+        }
+      }
+    }
+  }
+}
+
 // One tile of step d at outer offsets off, CNT = max(1, 2^nT / 256) outputs
 // per thread.  For each configuration of the outer reduced labels (Gray-code
 // order, one stride change each) the tile stages its sub-tensors of A and B
@@ -339,19 +498,6 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
   T acc[CNT];
 #pragma unroll
   for (int m = 0; m < CNT; m++) acc[m] = Trop<T>::zero();
-  // the lowest (up to 3) sub-tensor reduced bits unrolled in pairs (add +
-  // max3), the rest walked in Gray-code order
-  const int nR = d->nR;
-  const int nlo = nR < 3 ? nR : 3;
-  uint32_t lsA[4] = {0, 0, 0, 0}, lsB[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int b = 0; b < 3; b++)
-    if (b < nlo) {
-      lsA[b] = pos_stride(TBH_KF(d, rA)[b]);
-      lsB[b] = pos_stride(TBH_KF(d, rB)[b]);
-    }
-  const int nq = 1 << nlo;
-  const uint32_t nh = 1u << (nR - nlo);
   const uint32_t nc = 1u << d->nRo;
   uint32_t cA = 0, cB = 0;
   for (uint32_t c = 0; c < nc; c++) {
@@ -375,114 +521,7 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
     __builtin_amdgcn_s_waitcnt(0);
 #endif
     __syncthreads();
-    if (MA > 0 && active) {
-      constexpr int NA = 1 << MA, NB = 16 >> MA;
-      uint32_t ltA = e.loTA, ltB = e.loTB;
-      asm volatile("" : "+v"(ltA), "+v"(ltB));
-      uint32_t pA[NA], pB[NB];
-#pragma unroll
-      for (int j = 0; j < NA; j++) pA[j] = lds_pad(ltA + hi_at(e.hTA, j));
-#pragma unroll
-      for (int j = 0; j < NB; j++) pB[j] = lds_pad(ltB + hi_at(e.hTB, j << MA));
-      uint32_t rA = 0, rB = 0;
-      for (uint32_t h = 0; h < nh; h++) {
-        if (h) {
-          const int i = __builtin_ctz(h);
-          const uint32_t g = h ^ (h >> 1);
-          const uint32_t sa = pos_stride(TBH_KF(d, rA)[nlo + i]), sb = pos_stride(TBH_KF(d, rB)[nlo + i]);
-          if ((g >> i) & 1) {
-            rA += sa;
-            rB += sb;
-          } else {
-            rA -= sa;
-            rB -= sb;
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q += 2) {
-          if (q >= nq) break;
-          const uint32_t qa0 = lds_pad(rA + hi_at(lsA, q)), qb0 = lds_pad(rB + hi_at(lsB, q));
-          T a0[NA], b0[NB];
-#pragma unroll
-          for (int j = 0; j < NA; j++) a0[j] = lA[pA[j] + qa0];
-#pragma unroll
-          for (int j = 0; j < NB; j++) b0[j] = lB[pB[j] + qb0];
-          if (nq > 1) {
-            const uint32_t qa1 = lds_pad(rA + hi_at(lsA, q + 1));
-            const uint32_t qb1 = lds_pad(rB + hi_at(lsB, q + 1));
-            T a1[NA], b1[NB];
-#pragma unroll
-            for (int j = 0; j < NA; j++) a1[j] = lA[pA[j] + qa1];
-#pragma unroll
-            for (int j = 0; j < NB; j++) b1[j] = lB[pB[j] + qb1];
-#pragma unroll
-            for (int mb = 0; mb < NB; mb++)
-#pragma unroll
-              for (int ma = 0; ma < NA; ma++)
-                acc[ma + NA * mb] = max3(acc[ma + NA * mb], Trop<T>::mul(a0[ma], b0[mb]),
-                                         Trop<T>::mul(a1[ma], b1[mb]));
-          } else {
-#pragma unroll
-            for (int mb = 0; mb < NB; mb++)
-#pragma unroll
-              for (int ma = 0; ma < NA; ma++)
-                acc[ma + NA * mb] = Trop<T>::add(acc[ma + NA * mb], Trop<T>::mul(a0[ma], b0[mb]));
-          }
-        }
-      }
-    } else if (active) {
-      // per-thread LDS addresses, recomputed per chunk: the empty asm hides
-      // that they are loop-invariant, so they are not held in registers
-      // across the staging
-      uint32_t ltA = e.loTA, ltB = e.loTB;
-      asm volatile("" : "+v"(ltA), "+v"(ltB));
-      uint32_t pA[CNT], pB[CNT];
-#pragma unroll
-      for (int m = 0; m < CNT; m++) {
-        pA[m] = lds_pad(ltA + hi_at(e.hTA, m));
-        if (HASB) pB[m] = lds_pad(ltB + hi_at(e.hTB, m));
-      }
-      uint32_t rA = 0, rB = 0;
-      for (uint32_t h = 0; h < nh; h++) {
-        if (h) {
-          const int i = __builtin_ctz(h);
-          const uint32_t g = h ^ (h >> 1);
-          const uint32_t sa = pos_stride(TBH_KF(d, rA)[nlo + i]), sb = pos_stride(TBH_KF(d, rB)[nlo + i]);
-          if ((g >> i) & 1) {
-            rA += sa;
-            rB += sb;
-          } else {
-            rA -= sa;
-            rB -= sb;
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q += 2) {
-          if (q >= nq) break;
-          const uint32_t qa0 = lds_pad(rA + hi_at(lsA, q)), qb0 = lds_pad(rB + hi_at(lsB, q));
-          if (nq > 1) {
-            const uint32_t qa1 = lds_pad(rA + hi_at(lsA, q + 1));
-            const uint32_t qb1 = lds_pad(rB + hi_at(lsB, q + 1));
-#pragma unroll
-            for (int m = 0; m < CNT; m++) {
-              T x0 = lA[pA[m] + qa0], x1 = lA[pA[m] + qa1];
-              if (HASB) {
-                x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
-                x1 = Trop<T>::mul(x1, lB[pB[m] + qb1]);
-              }
-              acc[m] = max3(acc[m], x0, x1);
-            }
-          } else {
-#pragma unroll
-            for (int m = 0; m < CNT; m++) {
-              T x0 = lA[pA[m] + qa0];
-              if (HASB) x0 = Trop<T>::mul(x0, lB[pB[m] + qb0]);
-              acc[m] = Trop<T>::add(acc[m], x0);
-            }
-          }
-        }
-      }
-    }
+    if (active) ew_compute<T, HASB, CNT, MA>(d, e, lA, lB, acc);
     __syncthreads();
   }
   // contiguous writes along C's lowest positions
@@ -712,6 +751,241 @@ __global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW
       else
         ew_tile_dispatch<T, false>(d, off, e, lA, lB);
     }
+  }
+}
+
+// ------------------------------------------------- K2, prefetching ----
+// tbh_ew_pf_kernel: the LDS-tiled EW step with the next unit's sub-tensors
+// loaded into registers while the current unit is computed.  A unit is one
+// staging of a tile (one configuration of its outer reduced labels).  One
+// LDS buffer pair, as tbh_ew_kernel, so four workgroups stay resident per CU
+// (two buffer pairs, with two workgroups per CU, measured 1.7x slower: too
+// few waves hide the LDS reads of the compute).  Per unit:
+//   registers -> LDS; the previous tile's outputs stored; barrier;
+//   the next unit's global loads issued into the registers;
+//   compute from LDS; barrier.
+// The loads stay in flight through the compute because nothing there waits
+// on vmcnt: descriptors are read with scalar loads (constant address space),
+// the barriers are a bare s_barrier after an LDS wait (a fence would wait for
+// every memory operation), and a tile's outputs are stored only after the
+// next unit's registers have landed.  Only units of the same step are
+// prefetched: steps of >= 256 tiles hold nearly all of the work, and a
+// workgroup meets a step boundary at most once per step.
+
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// One operand's sub-tensor (2^nS entries) into registers: entry tid + 256 m
+// in v[m].  Every register is loaded, whatever nS: the strides of sub-tensor
+// bits at or above nS are 0, so the loads past the sub-tensor repeat
+// addresses inside it (L1 hits).  Loads under a guard would make each
+// register a merge of its old and new value, which the compiler resolves by
+// waiting for the load and copying -- the wait this kernel exists to avoid;
+// for the same reason there is one load form (dword: a 16-byte form beside
+// it made the compiler's wait analysis merge both forms' pending registers
+// and wait for every load before issuing the next unit's).
+template <typename T>
+__device__ __forceinline__ void pf_load(const TBH_GLOBAL T* __restrict__ X, uint32_t lo,
+                                        const uint32_t (&hS)[4], T (&v)[16]) {
+  // a uniform base plus a 32-bit byte offset per lane (operands hold at most
+  // 2^30 entries): global loads with an SGPR base, one VGPR of address each
+  const TBH_GLOBAL char* Xb = reinterpret_cast<const TBH_GLOBAL char*>(X);
+  // recomputed per call: the per-load offsets are not held across the unit loop
+  asm volatile("" : "+v"(lo));
+#pragma unroll
+  for (int m = 0; m < 16; m++)
+    v[m] = *reinterpret_cast<const TBH_GLOBAL T*>(Xb + (uint32_t)((lo + hi_at(hS, m)) * sizeof(T)));
+}
+
+// the registers of pf_load into the LDS (the layout of ew_stage; entries past
+// the sub-tensor land in the unused rest of the 2^12-entry buffer)
+template <typename T>
+__device__ __forceinline__ void pf_store(T* __restrict__ lX, const T (&v)[16]) {
+  uint32_t pt = lds_pad(threadIdx.x);
+  asm volatile("" : "+v"(pt));
+#pragma unroll
+  for (int m = 0; m < 16; m++) lX[pt + lds_pad(256) * m] = v[m];
+}
+
+// A workgroup's tiles: claims of `chunk` tiles, gridDim.x * chunk apart.
+// Every field is the same in all lanes; readfirstlane says so, which keeps
+// them (and the descriptor reads they index) scalar.
+struct PfWalk {
+  int64_t t, s, e, total;  // current tile, its claim's start and end
+  int64_t stride;
+  int chunk;
+  __device__ __forceinline__ void next() {
+    if (++t >= e) {
+      s += stride;
+      t = s;
+      e = s + chunk < total ? s + chunk : total;
+    }
+    t = uniform64(t);
+    s = uniform64(s);
+    e = uniform64(e);
+  }
+};
+
+__device__ __forceinline__ uint32_t uniform32(uint32_t x) {
+  return __builtin_amdgcn_readfirstlane(x);
+}
+
+// Every unit of the workgroup's tiles in step d (tiles [ob, oe)), from the
+// walk's current tile; returns with the walk at its first tile past the step.
+// The first unit is loaded before the loop, so every iteration starts with
+// its registers in flight from the previous one (a loop entry without them
+// would make the compiler wait on them where the next loads are issued).
+template <typename T, bool HASB, int CNT, int MA>
+__device__ __forceinline__ void pf_run(KEwS d, int64_t ob, int64_t oe, const EwThread& e,
+                                       PfWalk& w, T* __restrict__ lA, T* __restrict__ lB) {
+  const int tid = threadIdx.x;
+  const bool active = CNT > 1 || tid < (1 << d->nT);
+  const TBH_GLOBAL T* A0 = reinterpret_cast<const TBH_GLOBAL T*>(d->a);
+  const TBH_GLOBAL T* B0 = reinterpret_cast<const TBH_GLOBAL T*>(d->b);
+  TBH_GLOBAL T* C0 = reinterpret_cast<TBH_GLOBAL T*>(d->c);
+  const uint32_t nc = 1u << d->nRo;
+  // the unit in the registers: tile t at outer offsets off, configuration c
+  // of its outer reduced labels
+  int64_t t = w.t;
+  EwOuter off = outer_offsets(d, uniform32((uint32_t)(t - ob)));
+  uint32_t c = 0, cA = 0, cB = 0;
+  T va[16], vb[16];
+  pf_load<T>(A0 + off.a, e.loSA, e.hSA, va);
+  if (HASB) pf_load<T>(B0 + off.b, e.loSB, e.hSB, vb);
+  T acc[CNT];
+#pragma unroll
+  for (int m = 0; m < CNT; m++) acc[m] = Trop<T>::zero();
+  bool pend = false;  // acc holds a finished tile (outputs at pC)
+  uint32_t pC = 0;
+  for (;;) {
+    pf_store<T>(lA, va);
+    if (HASB) pf_store<T>(lB, vb);
+    if (pend) {
+      if (active) {
+        uint32_t lc = e.loTC;
+        asm volatile("" : "+v"(lc));
+#pragma unroll
+        for (int m = 0; m < CNT; m++) C0[pC + lc + hi_at(e.hTC, m)] = Trop<T>::canon(acc[m]);
+      }
+#pragma unroll
+      for (int m = 0; m < CNT; m++) acc[m] = Trop<T>::zero();
+      pend = false;
+    }
+    lds_barrier();
+    // the next unit: the tile's next configuration, or the next tile
+    int64_t tn = t;
+    uint32_t cn = 0, cAn = 0, cBn = 0;
+    EwOuter offn = off;
+    bool more = true;
+    if (c + 1 < nc) {
+      cn = c + 1;
+      const int i = __builtin_ctz(cn);
+      const uint32_t g = cn ^ (cn >> 1);
+      const uint32_t sa = pos_stride(TBH_KF(d, qA)[i]), sb = pos_stride(TBH_KF(d, qB)[i]);
+      cAn = ((g >> i) & 1) ? cA + sa : cA - sa;
+      cBn = ((g >> i) & 1) ? cB + sb : cB - sb;
+    } else {
+      w.next();
+      tn = w.t;
+      more = tn < oe;
+      if (more) {
+        const uint32_t o = uniform32((uint32_t)(tn - ob));
+        if (tn == t + 1)
+          outer_step(d, o, offn);
+        else
+          offn = outer_offsets(d, o);
+      }
+    }
+    if (more) {
+      pf_load<T>(A0 + offn.a + cAn, e.loSA, e.hSA, va);
+      if (HASB) pf_load<T>(B0 + offn.b + cBn, e.loSB, e.hSB, vb);
+    }
+    if (active) ew_compute<T, HASB, CNT, MA>(d, e, lA, lB, acc);
+    if (c + 1 == nc) {
+      pend = true;
+      pC = off.c;
+    }
+    lds_barrier();
+    if (!more) break;
+    t = tn;
+    c = cn;
+    cA = cAn;
+    cB = cBn;
+    off = offn;
+  }
+  if (active) {  // the last unit always ends a tile
+    uint32_t lc = e.loTC;
+    asm volatile("" : "+v"(lc));
+#pragma unroll
+    for (int m = 0; m < CNT; m++) C0[pC + lc + hi_at(e.hTC, m)] = Trop<T>::canon(acc[m]);
+  }
+}
+
+template <typename T, bool HASB>
+__device__ __forceinline__ void pf_dispatch(KEwS d, int64_t ob, int64_t oe, const EwThread& e,
+                                            PfWalk& w, T* __restrict__ lA, T* __restrict__ lB) {
+  switch (d->nT <= 8 ? 0 : d->nT - 8) {
+    case 0: pf_run<T, HASB, 1, 0>(d, ob, oe, e, w, lA, lB); break;
+    case 1: pf_run<T, HASB, 2, 0>(d, ob, oe, e, w, lA, lB); break;
+    case 2: pf_run<T, HASB, 4, 0>(d, ob, oe, e, w, lA, lB); break;
+    case 3: pf_run<T, HASB, 8, 0>(d, ob, oe, e, w, lA, lB); break;
+    default:
+      if constexpr (HASB) {
+        switch (d->mlay) {
+          case 1: pf_run<T, true, 16, 1>(d, ob, oe, e, w, lA, lB); return;
+          case 2: pf_run<T, true, 16, 2>(d, ob, oe, e, w, lA, lB); return;
+          case 3: pf_run<T, true, 16, 3>(d, ob, oe, e, w, lA, lB); return;
+          default: break;
+        }
+      }
+      pf_run<T, HASB, 16, 0>(d, ob, oe, e, w, lA, lB);
+      break;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW_WAVES))) void
+tbh_ew_pf_kernel(const EwDesc* __restrict__ ops_g, const int64_t* __restrict__ prefix_g, int nops,
+                 int64_t total, int chunk) {
+  __shared__ __align__(16) T lA[kLdsElems];
+  __shared__ __align__(16) T lB[kLdsElems];
+  const KEwS ops = (KEwS)ops_g;
+  const __attribute__((address_space(4))) int64_t* prefix =
+      (const __attribute__((address_space(4))) int64_t*)prefix_g;
+  PfWalk w;
+  w.total = total;
+  w.chunk = chunk;
+  w.stride = (int64_t)gridDim.x * chunk;
+  w.s = (int64_t)blockIdx.x * chunk;
+  w.t = w.s;
+  w.e = w.s + chunk < total ? w.s + chunk : total;
+  int op = 0;
+  EwThread e;
+  while (w.t < total) {
+    // the step holding tile w.t (forward from the current one: galloping,
+    // then bisection; prefix[nops] = total > w.t)
+    int lo = op, hi = op + 1, step = 1;
+    while (hi < nops && prefix[hi] <= w.t) {
+      lo = hi;
+      step <<= 1;
+      hi = lo + step;
+    }
+    if (hi > nops) hi = nops;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (prefix[mid] <= w.t)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    op = __builtin_amdgcn_readfirstlane(lo);  // uniform: descriptors by scalar loads
+    const KEwS d = ops + op;
+    ew_thread_setup(d, e);
+    if (d->b != 0)
+      pf_dispatch<T, true>(d, prefix[op], prefix[op + 1], e, w, lA, lB);
+    else
+      pf_dispatch<T, false>(d, prefix[op], prefix[op + 1], e, w, lA, lB);
   }
 }
 
@@ -1214,6 +1488,25 @@ int ew_blocks_per_cu(int dtype) {
   const hipError_t err =
       dtype == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tbh_ew_kernel<float>, kEwBlock, 0)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tbh_ew_kernel<int32_t>, kEwBlock, 0);
+  return err == hipSuccess && n > 0 ? n : 4;
+}
+
+hipError_t launch_ew_pf(int dtype, const EwDesc* ops, const int64_t* prefix, int nops,
+                        int64_t total, int grid, int chunk, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  if (dtype == 0)
+    tbh_ew_pf_kernel<float><<<grid, kEwBlock, 0, s>>>(ops, prefix, nops, total, chunk);
+  else
+    tbh_ew_pf_kernel<int32_t><<<grid, kEwBlock, 0, s>>>(ops, prefix, nops, total, chunk);
+  return hipGetLastError();
+}
+
+int ew_pf_blocks_per_cu(int dtype) {
+  int n = 0;
+  const hipError_t err =
+      dtype == 0
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tbh_ew_pf_kernel<float>, kEwBlock, 0)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tbh_ew_pf_kernel<int32_t>, kEwBlock, 0);
   return err == hipSuccess && n > 0 ? n : 4;
 }
 

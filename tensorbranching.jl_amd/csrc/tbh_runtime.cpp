@@ -250,6 +250,9 @@ tbh_tuning default_tuning() {
     d.direct_max_reduce = o.direct_max_reduce;
     d.concurrent = 1;
     d.first_batch = 0.15;
+    d.ew_prefetch = 0;
+    d.ew_grid_mult = 16;
+    d.gemm_grid_mult = 16;
     if (const char* e = std::getenv("TBH_CHAIN"))
       std::sscanf(e, "%d,%d,%d", &d.chain_big, &d.chain_side, &d.chain_reduce);
     if (const char* e = std::getenv("TBH_SPLITK"))
@@ -263,6 +266,11 @@ tbh_tuning default_tuning() {
       d.ew_per_cu = std::min(8, std::max(1, std::atoi(e)));
     if (const char* e = std::getenv("TBH_FIRST_BATCH"))
       d.first_batch = std::min(1.0, std::max(0.01, std::atof(e)));
+    if (const char* e = std::getenv("TBH_EW_PF")) d.ew_prefetch = std::atoi(e) != 0;
+    if (const char* e = std::getenv("TBH_EW_GRID_MULT"))
+      d.ew_grid_mult = std::min(64, std::max(1, std::atoi(e)));
+    if (const char* e = std::getenv("TBH_GEMM_GRID_MULT"))
+      d.gemm_grid_mult = std::min(64, std::max(1, std::atoi(e)));
     return d;
   }();
   return t;
@@ -495,23 +503,30 @@ GemmDesc make_gemm(const POp& op, const std::vector<PTensor>& T, uint64_t base) 
 
 int64_t gemm_tiles(const GemmDesc& d) { return (int64_t)d.tiles_m * d.tiles_n * d.batch; }
 
-// EW workgroups: one tile chunk each, ew_per_cu per CU but no more than are
-// resident at once (5: 32 KB of LDS each).  4 measured faster than 5 on the
-// 3-regular n=300 slices.
+// EW workgroups: claims of `chunk` tiles, ew_per_cu resident per CU (no more
+// than fit: 5 by the 32 KB of LDS each; 4 measured faster than 5 on the
+// 3-regular n=300 slices), times ew_grid_mult.  A grid of 16x the resident
+// workgroups measured faster than a persistent one (tiled EW 157 -> 153 ms
+// per n=300 solve): workgroups that start as others finish even out claims
+// of uneven cost.
 int ew_grid(tbh_ctx* ctx, int dtype, int64_t total, int chunk) {
   int64_t blocks = (total + chunk - 1) / chunk;
-  const int per_cu = std::min(ctx->tune.ew_per_cu, ctx->ew_occ[dtype ? 1 : 0]);
+  const int occ = ctx->tune.ew_prefetch ? ctx->ew_pf_occ[dtype ? 1 : 0] : ctx->ew_occ[dtype ? 1 : 0];
+  const int64_t per_cu = (int64_t)std::min(ctx->tune.ew_per_cu, occ) * ctx->tune.ew_grid_mult;
   return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)ctx->num_cus * per_cu));
 }
 
-// a persistent grid of as many workgroups as stay resident: a larger grid
-// leaves workgroups that start only when others have finished their tiles
+// GEMM workgroups: the resident workgroups per CU (occupancy query) times
+// gemm_grid_mult.  16x measured faster than a persistent grid (GEMM 161 ->
+// 138 ms per n=300 solve): the tile costs of a level's GEMMs differ (K from
+// 2^5 up), and late-starting workgroups balance them.
 int gemm_grid(tbh_ctx* ctx, int64_t total, int dtype, bool vec, int shape) {
   static const int forced =
       std::getenv("TBH_GEMM_PER_CU") ? std::atoi(std::getenv("TBH_GEMM_PER_CU")) : 0;
   int per_cu = forced > 0 ? forced : ctx->gemm_occ[dtype ? 1 : 0][vec ? 1 : 0][shape];
   if (per_cu <= 0) per_cu = 2;  // occupancy not queried (tbh_create fills it)
-  return (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)ctx->num_cus * per_cu));
+  const int64_t blocks = (int64_t)ctx->num_cus * per_cu * ctx->tune.gemm_grid_mult;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(total, blocks));
 }
 
 // direct steps: wave tiles, 4 waves per workgroup
@@ -827,7 +842,8 @@ int BatchRunner::enqueue(int dtype, const std::vector<const SliceOwned*>& slices
                               direct_grid(ctx, total, direct_chunk(ctx, total)),
                               direct_chunk(ctx, total), sk[0]);
     if (k == 1)
-      return launch_ew(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
+      return (ctx->tune.ew_prefetch ? launch_ew_pf : launch_ew)(
+          dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
                        ew_grid(ctx, dtype, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
                        sk[0]);
     return launch_gemm(dtype, true, k - 2, reinterpret_cast<const GemmDesc*>(descs), prefix, nops,
@@ -1089,7 +1105,8 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
                                direct_grid(ctx, tiles, direct_chunk(ctx, tiles)),
                                direct_chunk(ctx, tiles), ctx->stream);
       else
-        err = launch_ew(dtype, (const EwDesc*)(dbase + q.o_op[k]), pre, 1, tiles,
+        err = (ctx->tune.ew_prefetch ? launch_ew_pf : launch_ew)(
+            dtype, (const EwDesc*)(dbase + q.o_op[k]), pre, 1, tiles,
                         ew_grid(ctx, dtype, tiles, ew_chunk(ctx, tiles)), ew_chunk(ctx, tiles),
                         ctx->stream);
       ctx->stats.ew_ops++;
@@ -1192,6 +1209,7 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
   ctx->num_cus = prop.multiProcessorCount;
   for (int t = 0; t < 2; t++) {
     ctx->ew_occ[t] = ew_blocks_per_cu(t);
+    ctx->ew_pf_occ[t] = ew_pf_blocks_per_cu(t);
     for (int v = 0; v < 2; v++)
       for (int s = 0; s < 3; s++) ctx->gemm_occ[t][v][s] = gemm_blocks_per_cu(t, v != 0, s);
   }
@@ -1633,7 +1651,8 @@ static int check_tuning(const tbh_tuning& t) {
       t.direct_chunk < 1 || t.direct_chunk > 64 || t.ew_per_cu < 1 || t.ew_per_cu > 8 ||
       t.direct_max_reduce < -1 || t.direct_max_reduce > kMaxReduce || t.concurrent < 0 ||
       t.concurrent > 1 ||
-      !(t.first_batch > 0.0 && t.first_batch <= 1.0)) {
+      !(t.first_batch > 0.0 && t.first_batch <= 1.0) || t.ew_prefetch < 0 || t.ew_prefetch > 1 ||
+      t.ew_grid_mult < 1 || t.ew_grid_mult > 64 || t.gemm_grid_mult < 1 || t.gemm_grid_mult > 64) {
     set_error("tuning parameter out of range");
     return TBH_EINVAL;
   }
@@ -1764,7 +1783,10 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
       {{2, 3, 4, 5}, nullptr, &tbh_tuning::ew_per_cu},
       {{-1, 0, 1, 2, 3}, nullptr, &tbh_tuning::direct_max_reduce},
       {{0, 1}, nullptr, &tbh_tuning::concurrent},
-      {{0.05, 0.1, 0.15, 0.25, 0.4}, &tbh_tuning::first_batch, nullptr}};
+      {{0.05, 0.1, 0.15, 0.25, 0.4}, &tbh_tuning::first_batch, nullptr},
+      {{0, 1}, nullptr, &tbh_tuning::ew_prefetch},
+      {{1, 4, 16, 64}, nullptr, &tbh_tuning::ew_grid_mult},
+      {{1, 4, 16, 64}, nullptr, &tbh_tuning::gemm_grid_mult}};
   // chain re-association regroups a chain's sums: for Float32 slices whose
   // sums are not all exact (non-integer weights or r, or sums reaching
   // 2^24) a result may move by a rounding step, so the chain parameters are

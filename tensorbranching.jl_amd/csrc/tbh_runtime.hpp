@@ -34,6 +34,10 @@ hipError_t launch_epilogue(int dtype, const EpiDesc* e, int64_t n, void* out, hi
 hipError_t launch_valu_probe(int dtype, void* out, int blocks, int iters, hipStream_t s);
 // resident tiled-EW workgroups per CU (LDS and registers), from the runtime
 int ew_blocks_per_cu(int dtype);
+// the same step with the next tile's sub-tensors prefetched into registers
+hipError_t launch_ew_pf(int dtype, const EwDesc* ops, const int64_t* prefix, int nops,
+                        int64_t total, int grid, int chunk, hipStream_t s);
+int ew_pf_blocks_per_cu(int dtype);
 // resident workgroups per CU of one GEMM kernel (dtype, VEC, tile shape)
 int gemm_blocks_per_cu(int dtype, bool vec, int shape);
 
@@ -131,7 +135,8 @@ struct Profiler {
 struct tbh_ctx {
   int device = 0;
   int num_cus = 256;
-  int ew_occ[2] = {4, 4};  // resident tiled-EW workgroups per CU, per element type
+  int ew_occ[2] = {4, 4};     // resident tiled-EW workgroups per CU, per element type
+  int ew_pf_occ[2] = {4, 4};  // the same for the prefetching kernel
   int gemm_occ[2][2][3] = {};  // resident GEMM workgroups per CU [dtype][vec][shape],
                                // from the occupancy query in tbh_create
   hipStream_t stream = nullptr;

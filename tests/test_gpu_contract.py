@@ -133,11 +133,14 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
     for mode in ("batched", "plain"):
         assert np.array_equal(backend.contract_slices(br, dtype, mode), ref), mode
     # direct wave tiles with reductions (every eligible step direct), no
-    # direct steps at all, and every launch on one stream (GEMMs not beside
-    # the EW steps of their level)
+    # direct steps at all, every launch on one stream (GEMMs not beside the
+    # EW steps of their level), and tiled EW steps staged by DMA after each
+    # tile instead of prefetched into registers (alone and with every EW
+    # step tiled)
     t0 = backend.tuning()
     try:
-        for kw in ({"direct_max_reduce": 12}, {"direct_max_reduce": -1}, {"concurrent": 0}):
+        for kw in ({"direct_max_reduce": 12}, {"direct_max_reduce": -1}, {"concurrent": 0},
+                   {"ew_prefetch": 0}, {"ew_prefetch": 0, "direct_max_reduce": -1}):
             backend.set_tuning(**dict(t0, **kw))
             assert np.array_equal(backend.contract_slices(br, dtype, "batched"), ref), kw
     finally:

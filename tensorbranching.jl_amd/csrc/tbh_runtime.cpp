@@ -244,7 +244,7 @@ tbh_tuning default_tuning() {
     d.chain_reduce = o.chain_reduce;
     d.split_tiles_bits = o.split_tiles_bits;
     d.split_min_k_bits = o.split_min_k_bits;
-    d.ew_chunk = 2;
+    d.ew_chunk = 4;
     d.direct_chunk = 4;
     d.ew_per_cu = 4;
     d.direct_max_reduce = o.direct_max_reduce;
@@ -537,7 +537,9 @@ int direct_grid(tbh_ctx* ctx, int64_t total, int chunk) {
 
 // Waves (direct) and workgroups (tiled) take small chunks of the tile space
 // round-robin, which spreads expensive steps evenly; long contiguous ranges
-// measured slower (costly steps cluster in a few ranges).
+// measured slower (costly steps cluster in a few ranges).  Tiled claims of 4
+// tiles measured best with the 16x grid (n=300 solve: 2 -> 291 ms, 4 -> 280,
+// 8 -> 279, 16 -> 287).
 int direct_chunk(tbh_ctx* ctx, int64_t) { return std::max(1, ctx->tune.direct_chunk); }
 int ew_chunk(tbh_ctx* ctx, int64_t) { return std::max(1, ctx->tune.ew_chunk); }
 

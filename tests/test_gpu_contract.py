@@ -134,13 +134,13 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
         assert np.array_equal(backend.contract_slices(br, dtype, mode), ref), mode
     # direct wave tiles with reductions (every eligible step direct), no
     # direct steps at all, every launch on one stream (GEMMs not beside the
-    # EW steps of their level), and tiled EW steps staged by DMA after each
-    # tile instead of prefetched into registers (alone and with every EW
-    # step tiled)
+    # EW steps of their level), tiled EW steps prefetched into registers
+    # (alone and with every EW step tiled), and persistent grids
     t0 = backend.tuning()
     try:
         for kw in ({"direct_max_reduce": 12}, {"direct_max_reduce": -1}, {"concurrent": 0},
-                   {"ew_prefetch": 0}, {"ew_prefetch": 0, "direct_max_reduce": -1}):
+                   {"ew_prefetch": 1}, {"ew_prefetch": 1, "direct_max_reduce": -1},
+                   {"ew_grid_mult": 1, "gemm_grid_mult": 1}):
             backend.set_tuning(**dict(t0, **kw))
             assert np.array_equal(backend.contract_slices(br, dtype, "batched"), ref), kw
     finally:

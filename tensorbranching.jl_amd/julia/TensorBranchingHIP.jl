@@ -122,10 +122,23 @@ struct TbhRequest          # tbh_request
     priority::Int32
 end
 
-start_serving(be::HIPBackend; max_resident_slices::Integer = 0) =
-    check(ccall((:tbh_serve_start, libtbh), Cint, (Ptr{Cvoid}, Csize_t), be.ctx,
-                max_resident_slices))
+struct TbhServeConfig       # tbh_serve_config
+    max_resident_slices::Csize_t   # slices per round, 0: no cap
+    round_log2_macs::Float64       # round work bound (log2 MACs), 0: default (38)
+end
+
+start_serving(be::HIPBackend; max_resident_slices::Integer = 0, round_log2_macs::Real = 0) =
+    check(ccall((:tbh_serve_start_ex, libtbh), Cint, (Ptr{Cvoid}, Ref{TbhServeConfig}), be.ctx,
+                TbhServeConfig(max_resident_slices, round_log2_macs)))
 stop_serving(be::HIPBackend) = check(ccall((:tbh_serve_stop, libtbh), Cint, (Ptr{Cvoid},), be.ctx))
+
+# (rounds, rounds run again request by request, requests that ended FAILED)
+function serve_stats(be::HIPBackend)
+    r, rr, f = Ref{UInt64}(0), Ref{UInt64}(0), Ref{UInt64}(0)
+    check(ccall((:tbh_serve_stats, libtbh), Cint,
+                (Ptr{Cvoid}, Ptr{UInt64}, Ptr{UInt64}, Ptr{UInt64}), be.ctx, r, rr, f))
+    return (rounds = r[], retried_rounds = rr[], failed_requests = f[])
+end
 
 function check(rc::Cint)
     rc == 0 || error(unsafe_string(ccall((:tbh_last_error, libtbh), Cstring, ())))
@@ -161,6 +174,65 @@ function wait_result(be::HIPBackend, ticket::UInt64, element_type::Type)
     status[] == 2 || error("request $ticket failed: " *
                            unsafe_string(ccall((:tbh_last_error, libtbh), Cstring, ())))
     return res[]
+end
+
+# ---- planner / launch parameters (tbh_tuning; results never depend on them)
+mutable struct TbhTuning
+    chain_big::Int32; chain_side::Int32; chain_reduce::Int32
+    split_tiles_bits::Int32; split_min_k_bits::Int32
+    ew_chunk::Int32; direct_chunk::Int32
+    ew_per_cu::Int32
+    direct_max_reduce::Int32
+    concurrent::Int32
+    first_batch::Float64
+    ew_prefetch::Int32
+    ew_grid_mult::Int32; gemm_grid_mult::Int32
+    TbhTuning() = new()
+end
+
+function get_tuning(be::HIPBackend)
+    t = TbhTuning()
+    check(ccall((:tbh_get_tuning, libtbh), Cint, (Ptr{Cvoid}, Ref{TbhTuning}), be.ctx, t))
+    return t
+end
+set_tuning!(be::HIPBackend, t::TbhTuning) =
+    check(ccall((:tbh_set_tuning, libtbh), Cint, (Ptr{Cvoid}, Ref{TbhTuning}), be.ctx, t))
+
+# ---- branching tables of regions (the table step of optimal_branches,
+# src/branch.jl:79): alpha holds, per region, the SizeMax value of each
+# boundary configuration (from contract_slices, one slice per configuration);
+# returns the reduced alpha and, per region, the table's configurations
+struct TbhRegion           # tbh_region
+    nv::Int32
+    adj::Ptr{UInt32}
+    weights::Ptr{Cvoid}
+    open::UInt32
+end
+
+function region_tables(be::HIPBackend, element_type::Type, adj::Vector{Vector{UInt32}},
+                       weights::Vector, open::Vector{UInt32}, alpha::Vector)
+    n = length(adj)
+    w = [isnothing(x) ? nothing : convert(Vector{element_type}, x) for x in weights]
+    a = convert(Vector{element_type}, copy(alpha))
+    group_ptr = zeros(Csize_t, n + 1)
+    total = Ref{Csize_t}(0)
+    GC.@preserve adj w begin
+        regs = [TbhRegion(Int32(length(adj[i])), pointer(adj[i]),
+                          isnothing(w[i]) ? C_NULL : pointer(w[i]), open[i]) for i in 1:n]
+        # sizing pass, then the configurations
+        check(ccall((:tbh_region_tables, libtbh), Cint,
+                    (Ptr{Cvoid}, Cint, Ptr{TbhRegion}, Csize_t, Ptr{Cvoid}, Ptr{Csize_t},
+                     Ptr{UInt32}, Csize_t, Ptr{Csize_t}),
+                    be.ctx, dtype_code(element_type), regs, n, copy(a), group_ptr, C_NULL, 0,
+                    total))
+        configs = zeros(UInt32, total[])
+        check(ccall((:tbh_region_tables, libtbh), Cint,
+                    (Ptr{Cvoid}, Cint, Ptr{TbhRegion}, Csize_t, Ptr{Cvoid}, Ptr{Csize_t},
+                     Ptr{UInt32}, Csize_t, Ptr{Csize_t}),
+                    be.ctx, dtype_code(element_type), regs, n, a, group_ptr, configs,
+                    length(configs), total))
+    end
+    return a, [configs[group_ptr[i]+1:group_ptr[i+1]] for i in 1:n]
 end
 
 # dynamic_ob_mis with the contraction step on the MI355X (src/dynamic_ob.jl:6-28)

@@ -65,9 +65,9 @@ def cpu_baseline(br, gpu_results, seconds, threads):
     (work per second) measured on its sample; the solve's time is the sum
     over strata.  The one-thread figure samples the same way with a quarter
     of the budget, skipping slices that would alone exceed the stratum's
-    share; a stratum left without a sample takes the rate of the nearest
-    sampled one (reported).  Every sampled result is checked equal to the
-    GPU's."""
+    share but always contracting at least the stratum's smallest slice, so
+    every stratum's rate is measured on one thread too.  Every sampled result
+    is checked equal to the GPU's."""
     from oracle import oracle
     import __graft_entry__
     tb = __graft_entry__.package()
@@ -85,13 +85,11 @@ def cpu_baseline(br, gpu_results, seconds, threads):
             if guide is not None:
                 # one thread: skip slices whose time alone, predicted from the
                 # stratum's rate on all threads over their count, exceeds the
-                # stratum's share of the budget (the lowest stratum keeps at
-                # least its smallest slice)
+                # stratum's share of the budget; a stratum keeps at least its
+                # smallest slice (strata are in ascending tc)
                 r1 = guide[si] / threads if guide[si] else None
                 keep = [k for k in pick if r1 and work[k] / r1 <= budget / len(strata)]
-                if not keep and si == 0:
-                    keep = [st[0]]
-                pick = keep
+                pick = keep if keep else [st[0]]
             t_used, w_used, i = 0.0, 0.0, 0
             while i < len(pick) and (i == 0 or t_used < budget / len(strata)):
                 chunk = pick[i:i + max(1, nthreads if tcs[pick[i]] < 33 else 1)]

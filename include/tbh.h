@@ -275,7 +275,10 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode,
  * previous one; ew_grid_mult / gemm_grid_mult: a tiled EW / GEMM launch's
  * grid holds this many times the workgroups that stay resident (1: a
  * persistent grid; more: later workgroups start as earlier ones finish,
- * which evens out uneven tiles).  Results never depend on them. */
+ * which evens out uneven tiles); gemm_tiles: GEMM tile shapes (0: 128 x 128
+ * only; 1: 256 x 64 / 256 x 32 tiles for outputs of M >= 256 rows and N <= 64
+ * / 32 columns; 2: 256 x 64 up to N = 128 too).  Results never depend on
+ * them. */
 typedef struct {
   int32_t chain_big, chain_side, chain_reduce;
   int32_t split_tiles_bits, split_min_k_bits;
@@ -286,6 +289,7 @@ typedef struct {
   double first_batch;
   int32_t ew_prefetch;
   int32_t ew_grid_mult, gemm_grid_mult;
+  int32_t gemm_tiles;
 } tbh_tuning;
 int tbh_get_tuning(tbh_ctx* ctx, tbh_tuning* out);
 int tbh_set_tuning(tbh_ctx* ctx, const tbh_tuning* t);

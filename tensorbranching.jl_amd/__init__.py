@@ -14,7 +14,7 @@ from .api import (  # noqa: F401
     SimpleGraph, SlicedBranch, UnitWeight, code_complexity, complexity, contract_slices,
     contraction_all_memory, contraction_peak_memory, default_backend, plan_steps, sc,
     slice_footprint,
-    solve_slice, tc)
+    solve_slice, tc, workload_key)
 from .workload import (  # noqa: F401
     ContractionTreeSlicer, dynamic_ob_mis, erdos_renyi, initialize_code, random_ksg,
     random_regular_graph, serving_requests, slice_graph, SERVING_MIX, ServingRequest)

@@ -13,6 +13,9 @@ CPU contraction, so usecuda=False raises instead of silently computing on
 the host.  Labels and leaf indices are 0-based here (Julia's are 1-based).
 """
 import ctypes as C
+import hashlib
+import json
+import os
 from collections import namedtuple
 
 import numpy as np
@@ -181,6 +184,32 @@ def _dtype_code(element_type):
                     "tropical values are integer-valued and exact in 32 bits")
 
 
+def workload_key(branches, element_type=np.float32):
+    """A digest of a slice set as the backend sees it (element type, and per
+    slice nv, weights, r, leaf label lists and tree): the key of the
+    autotuner's cache."""
+    _, npdt = _dtype_code(element_type)
+    h = hashlib.sha256(np.dtype(npdt).name.encode())
+    for b in branches:
+        h.update(np.array([b.p.g.nv], np.int64).tobytes())
+        h.update(np.array([float(b.r)], np.float64).tobytes())
+        if b.p.weights is not None:
+            h.update(b"w" + np.ascontiguousarray(np.asarray(b.p.weights).astype(npdt)).tobytes())
+        if b.code is not None:
+            c = b.code
+            for a in (c.ix_ptr, c.ix, c.tree):
+                h.update(b"|" + np.ascontiguousarray(a, dtype=np.int32).tobytes())
+    return h.hexdigest()
+
+
+def _read_tuning_cache(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except FileNotFoundError:
+        return {}
+
+
 class _Marshal:
     """tbh_slice array over the branches, keeping its buffers alive."""
 
@@ -290,16 +319,36 @@ class HIPBackend:
             setattr(t, k, v)
         check(_lib.tbh().tbh_set_tuning(self.handle, C.byref(t)))
 
-    def autotune(self, branches, element_type=np.float32, max_trials=16):
+    def autotune(self, branches, element_type=np.float32, max_trials=16, cache=None):
         """Times BATCHED contractions of `branches` under candidate tunings,
-        installs the fastest; returns (tuning dict, ms of one contraction)."""
+        installs the fastest; returns (tuning dict, ms of one contraction).
+
+        cache: path of a JSON file of earlier results on this machine, keyed
+        by the workload (element type, trial budget and a digest of every
+        slice): a hit installs the stored tuning without timing anything; a
+        miss tunes and adds the result to the file."""
+        key = None
+        if cache is not None:
+            key = f"{workload_key(branches, element_type)}:{int(max_trials)}"
+            stored = _read_tuning_cache(cache).get(key)
+            if stored is not None:
+                self.set_tuning(**stored["tuning"])
+                return self.tuning(), float(stored["ms"])
         code, npdt = _dtype_code(element_type)
         m = _Marshal(branches, npdt)
         t = _lib.TbhTuning()
         ms = C.c_double()
         check(_lib.tbh().tbh_autotune(self.handle, code, m.ptr, m.n, int(max_trials), C.byref(t),
                                       C.byref(ms)))
-        return {k: getattr(t, k) for k, _ in t._fields_}, ms.value
+        best = {k: getattr(t, k) for k, _ in t._fields_}
+        if cache is not None:
+            table = _read_tuning_cache(cache)
+            table[key] = {"tuning": best, "ms": ms.value}
+            tmp = f"{cache}.tmp{os.getpid()}"
+            with open(tmp, "w") as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
+        return best, ms.value
 
     def valu_peak(self, element_type=np.float32):
         """Measured tropical ops/s of the GEMM's VALU instruction mix with

@@ -23,11 +23,14 @@ constexpr int kSubBits = 12;     // an LDS sub-tensor has at most 2^12 entries
 constexpr int kLdsElems =
     (1 << kSubBits) + (TBH_LDS_PAD_SHIFT ? ((1 << kSubBits) >> TBH_LDS_PAD_SHIFT) : 0);
 
-// GEMM tile shape for an M x N output: tall-skinny outputs (N <= 64, M >=
-// 256) take a 256-row tile so that a narrow N still fills it.
-__host__ __device__ inline int gemm_shape(int64_t M, int64_t N) {
-  if (M >= 256 && N <= 32) return 2;
-  if (M >= 256 && N <= 64) return 1;
+// GEMM tile shape for an M x N output under a tile rule: 1 (default):
+// tall-skinny outputs (N <= 64, M >= 256) take a 256-row tile so that a
+// narrow N still fills it; 0: 128 x 128 tiles only; 2: 256 x 64 tiles up to
+// N = 128 as well.
+__host__ __device__ inline int gemm_shape(int64_t M, int64_t N, int rule = 1) {
+  if (rule <= 0 || M < 256) return 0;
+  if (N <= 32) return 2;
+  if (N <= (rule >= 2 ? 128 : 64)) return 1;
   return 0;
 }
 __host__ __device__ inline int gemm_bm(int shape) { return shape ? 256 : 128; }

@@ -375,6 +375,7 @@ struct Builder {
     op.N = 1 << nb;
     op.K = 1 << nk;
     op.batch = 1 << nt;
+    op.shape = gemm_shape(op.M, op.N, opt.gemm_tiles);
     op.macs = std::ldexp(1.0, na + nb + nk + nt);
     op.bytes = 4.0 * (std::ldexp(1.0, na + nk + nt) + std::ldexp(1.0, nk + nb + nt) +
                       std::ldexp(1.0, na + nb + nt));
@@ -409,7 +410,7 @@ int gemm_scatter_mode(const PlanOptions& opt, const std::vector<int32_t>& L,
   if (opt.gemm_scatter_bits < 0 || L.size() != G.size() || (int)G.size() > kGemmScatter - 8 ||
       na < 2)
     return 0;
-  const int shape = gemm_shape((int64_t)1 << na, (int64_t)1 << nb);
+  const int shape = gemm_shape((int64_t)1 << na, (int64_t)1 << nb, opt.gemm_tiles);
   const int bm = __builtin_ctz(gemm_bm(shape)), bn = __builtin_ctz(gemm_bn(shape));
   const int vm = std::min(na, bm), vn = std::min(nb, bn);
   const int split = vm + vn > 13 ? 1 : 0;  // an LDS pass holds kEpiElems = 2^13 outputs
@@ -565,6 +566,7 @@ int verify_plan(const SlicePlan& P, std::string* err) {
           ec != ((int64_t)1 << rc))
         return bad("GEMM extents");
       if (op.M % 4 || op.K % 4) return bad("GEMM vector alignment");
+      if (op.shape < 0 || op.shape > 2) return bad("GEMM tile shape");
       if (op.scatter) {
         // pc: a bijection of the output bits onto C's positions, -1 past
         // them; scatter 1: rows 0 and 1 at positions 0 and 1 (quad stores);
@@ -576,7 +578,7 @@ int verify_plan(const SlicePlan& P, std::string* err) {
         if (op.scatter == 1 && (op.pc[0] != 0 || op.pc[1] != 1))
           return bad("GEMM output map: rows 0, 1");
         if (op.scatter == 2) {
-          const int shape = gemm_shape(op.M, op.N);
+          const int shape = op.shape;
           const int vm = std::min(na, __builtin_ctz(gemm_bm(shape)));
           const int vn = std::min(nb, __builtin_ctz(gemm_bn(shape)));
           if (vm < 2 || op.ep_split != (vm + vn > 13 ? 1 : 0) || op.ep_n != vm + vn - op.ep_split ||

@@ -43,6 +43,7 @@ struct POp {
   EwDesc ew{};
   // GEMM
   int32_t M = 0, N = 0, K = 0, batch = 0;
+  int shape = 0;                 // tile shape (gemm_shape under the plan's tile rule)
   int scatter = 0;               // GemmDesc::scatter: C in its own label order, GEMM
   int8_t pc[kGemmScatter] = {};  // output bit q (m, then n, then batch bits) at pc[q]
   int8_t ep[16] = {};            // scatter 2: a pass's tile bits in ascending C position
@@ -94,6 +95,9 @@ struct PlanOptions {
   // direct (wave-tile) EW steps reduce at most this many labels (-1: none
   // are direct); steps with reductions run faster LDS-tiled (DMA staging)
   int direct_max_reduce = 0;
+  // GEMM tile rule (gemm_shape): 0 128 x 128 only, 1 tall-skinny tiles for
+  // N <= 64, 2 also 256 x 64 up to N = 128
+  int gemm_tiles = 1;
 };
 
 // Returns 0 or sets *err and returns -1.

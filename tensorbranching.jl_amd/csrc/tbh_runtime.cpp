@@ -253,6 +253,7 @@ tbh_tuning default_tuning() {
     d.ew_prefetch = 0;
     d.ew_grid_mult = 16;
     d.gemm_grid_mult = 16;
+    d.gemm_tiles = 1;
     if (const char* e = std::getenv("TBH_CHAIN"))
       std::sscanf(e, "%d,%d,%d", &d.chain_big, &d.chain_side, &d.chain_reduce);
     if (const char* e = std::getenv("TBH_SPLITK"))
@@ -271,6 +272,8 @@ tbh_tuning default_tuning() {
       d.ew_grid_mult = std::min(64, std::max(1, std::atoi(e)));
     if (const char* e = std::getenv("TBH_GEMM_GRID_MULT"))
       d.gemm_grid_mult = std::min(64, std::max(1, std::atoi(e)));
+    if (const char* e = std::getenv("TBH_GEMM_TILES"))
+      d.gemm_tiles = std::min(2, std::max(0, std::atoi(e)));
     return d;
   }();
   return t;
@@ -291,6 +294,7 @@ PlanOptions plan_options(const tbh_ctx* ctx, bool batched) {
   }();
   o.gemm_scatter_bits = scatter_bits;
   o.direct_max_reduce = t.direct_max_reduce;
+  o.gemm_tiles = t.gemm_tiles;
   return o;
 }
 
@@ -490,7 +494,7 @@ GemmDesc make_gemm(const POp& op, const std::vector<PTensor>& T, uint64_t base) 
   d.sA = (int64_t)op.M * op.K;
   d.sB = (int64_t)op.K * op.N;
   d.sC = (int64_t)op.M * op.N;
-  d.shape = gemm_shape(op.M, op.N);
+  d.shape = op.shape;
   d.scatter = op.scatter;
   std::memcpy(d.pc, op.pc, sizeof d.pc);
   std::memcpy(d.ep, op.ep, sizeof d.ep);
@@ -551,11 +555,11 @@ namespace {
 // launch kinds: 0 direct EW, 1 tiled EW, 2 + s GEMM with tile shape s
 constexpr int kKinds = 5;
 int op_kind(const POp& op) {
-  return op.kind == kGemm ? 2 + gemm_shape(op.M, op.N) : (op.direct ? 0 : 1);
+  return op.kind == kGemm ? 2 + op.shape : (op.direct ? 0 : 1);
 }
 int64_t op_tiles(const POp& op) {
   if (op.kind != kGemm) return op.tiles;
-  const int sh = gemm_shape(op.M, op.N), bm = gemm_bm(sh), bn = gemm_bn(sh);
+  const int sh = op.shape, bm = gemm_bm(sh), bn = gemm_bn(sh);
   return (int64_t)((op.M + bm - 1) / bm) * ((op.N + bn - 1) / bn) * op.batch;
 }
 size_t desc_size(int kind) { return kind >= 2 ? sizeof(GemmDesc) : sizeof(EwDesc); }
@@ -1114,8 +1118,8 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
       ctx->stats.ew_ops++;
       ctx->stats.ew_ops_algo_bytes += op.bytes;
     } else {
-      err = launch_gemm(dtype, true, gemm_shape(op.M, op.N), (const GemmDesc*)(dbase + q.o_op[k]),
-                        pre, 1, tiles, gemm_grid(ctx, tiles, dtype, true, gemm_shape(op.M, op.N)),
+      err = launch_gemm(dtype, true, op.shape, (const GemmDesc*)(dbase + q.o_op[k]),
+                        pre, 1, tiles, gemm_grid(ctx, tiles, dtype, true, op.shape),
                         ctx->stream);
       ctx->stats.gemm_ops++;
       ctx->stats.gemm_ops_algo_ops += 2 * op.macs;
@@ -1654,7 +1658,8 @@ static int check_tuning(const tbh_tuning& t) {
       t.direct_max_reduce < -1 || t.direct_max_reduce > kMaxReduce || t.concurrent < 0 ||
       t.concurrent > 1 ||
       !(t.first_batch > 0.0 && t.first_batch <= 1.0) || t.ew_prefetch < 0 || t.ew_prefetch > 1 ||
-      t.ew_grid_mult < 1 || t.ew_grid_mult > 64 || t.gemm_grid_mult < 1 || t.gemm_grid_mult > 64) {
+      t.ew_grid_mult < 1 || t.ew_grid_mult > 64 || t.gemm_grid_mult < 1 || t.gemm_grid_mult > 64 ||
+      t.gemm_tiles < 0 || t.gemm_tiles > 2) {
     set_error("tuning parameter out of range");
     return TBH_EINVAL;
   }
@@ -1788,7 +1793,8 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
       {{0.05, 0.1, 0.15, 0.25, 0.4}, &tbh_tuning::first_batch, nullptr},
       {{0, 1}, nullptr, &tbh_tuning::ew_prefetch},
       {{1, 4, 16, 64}, nullptr, &tbh_tuning::ew_grid_mult},
-      {{1, 4, 16, 64}, nullptr, &tbh_tuning::gemm_grid_mult}};
+      {{1, 4, 16, 64}, nullptr, &tbh_tuning::gemm_grid_mult},
+      {{0, 1, 2}, nullptr, &tbh_tuning::gemm_tiles}};
   // chain re-association regroups a chain's sums: for Float32 slices whose
   // sums are not all exact (non-integer weights or r, or sums reaching
   // 2^24) a result may move by a rounding step, so the chain parameters are

@@ -187,6 +187,7 @@ mutable struct TbhTuning
     first_batch::Float64
     ew_prefetch::Int32
     ew_grid_mult::Int32; gemm_grid_mult::Int32
+    gemm_tiles::Int32
     TbhTuning() = new()
 end
 

@@ -172,9 +172,28 @@ def test_serving_entry_points_reject_null_context(tb):
 def test_tuning_struct_and_null_context(tb):
     from tensorbranching_jl_amd import _lib
     lib = _lib.tbh()
-    assert C.sizeof(_lib.TbhTuning) == 64  # 10 int32, a double, 3 int32, padding
+    assert C.sizeof(_lib.TbhTuning) == 64  # 10 int32, a double, 4 int32
     t = _lib.TbhTuning()
     assert lib.tbh_get_tuning(None, C.byref(t)) == _lib.TBH_EINVAL
     assert lib.tbh_set_tuning(None, C.byref(t)) == _lib.TBH_EINVAL
     ms = C.c_double()
     assert lib.tbh_autotune(None, 0, None, 0, 4, C.byref(t), C.byref(ms)) == _lib.TBH_EINVAL
+
+
+def test_workload_key_tracks_every_slice_field(tb):
+    """The autotuner cache key: equal for the same slice set, different when
+    a weight, r, the element type or the slice order changes."""
+    g = tb.random_regular_graph(40, 3, seed=4)
+    br = tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=6))
+    assert len(br) >= 2
+    k = tb.workload_key(br, np.float32)
+    assert k == tb.workload_key(list(br), np.float32)
+    assert k != tb.workload_key(br, np.int32)
+    assert k != tb.workload_key(br[::-1], np.float32)
+    b0 = br[0]
+    shifted = tb.SlicedBranch(b0.p, b0.code, b0.r + 1)
+    assert k != tb.workload_key([shifted] + list(br[1:]), np.float32)
+    w = np.ones(b0.p.g.nv, np.float32)
+    w[0] = 2.0
+    weighted = tb.SlicedBranch(tb.MISProblem(b0.p.g, w), b0.code, b0.r)
+    assert k != tb.workload_key([weighted] + list(br[1:]), np.float32)

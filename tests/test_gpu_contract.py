@@ -140,7 +140,8 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
     try:
         for kw in ({"direct_max_reduce": 12}, {"direct_max_reduce": -1}, {"concurrent": 0},
                    {"ew_prefetch": 1}, {"ew_prefetch": 1, "direct_max_reduce": -1},
-                   {"ew_grid_mult": 1, "gemm_grid_mult": 1}):
+                   {"ew_grid_mult": 1, "gemm_grid_mult": 1}, {"gemm_tiles": 0},
+                   {"gemm_tiles": 2}):
             backend.set_tuning(**dict(t0, **kw))
             assert np.array_equal(backend.contract_slices(br, dtype, "batched"), ref), kw
     finally:
@@ -218,11 +219,38 @@ def test_autotune_keeps_results(tb, orc):
         assert best["chain_side"] < best["chain_big"]
         assert np.array_equal(be.contract_slices(br, np.float32), orc.contract_slices(br, np.float32))
         for bad in ({"ew_chunk": 0}, {"ew_per_cu": 9}, {"direct_max_reduce": 13},
-                    {"concurrent": 2}, {"first_batch": 0.0}):
+                    {"concurrent": 2}, {"first_batch": 0.0}, {"gemm_tiles": 3},
+                    {"ew_grid_mult": 0}, {"ew_prefetch": 2}):
             with pytest.raises(tb.TbhError):
                 be.set_tuning(**bad)
         be.set_tuning(**t0)
         assert be.tuning() == t0
+    finally:
+        be.close()
+
+
+def test_autotune_cache(tb, orc, tmp_path):
+    """A cached autotune result is installed without timing: the second call
+    returns the first call's tuning and time, on a fresh context too; another
+    slice set misses and adds its own entry."""
+    path = str(tmp_path / "tuning.json")
+    g = tb.random_regular_graph(100, 3, seed=0)
+    br = _slices(tb, g, 10)
+    be = tb.HIPBackend(0, pool_bytes=2 << 30)
+    try:
+        best, ms = be.autotune(br, np.float32, max_trials=5, cache=path)
+    finally:
+        be.close()
+    be = tb.HIPBackend(0, pool_bytes=2 << 30)
+    try:
+        be.set_tuning(**dict(be.tuning(), ew_chunk=1))
+        best2, ms2 = be.autotune(br, np.float32, max_trials=5, cache=path)
+        assert best2 == best and ms2 == ms and be.tuning() == best
+        assert np.array_equal(be.contract_slices(br, np.float32), orc.contract_slices(br, np.float32))
+        other = _slices(tb, tb.random_regular_graph(80, 3, seed=3), 8)
+        be.autotune(other, np.float32, max_trials=3, cache=path)
+        import json
+        assert len(json.load(open(path))) == 2
     finally:
         be.close()
 

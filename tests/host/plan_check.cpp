@@ -36,12 +36,14 @@ int main(int argc, char** argv) {
     in.ix = ix.data();
     in.tree = tree.empty() ? nullptr : tree.data();
     in.unit_wbits = 0x3f800000u;  // Float32 1
-    std::vector<tbh::PlanOptions> variants(5);
+    std::vector<tbh::PlanOptions> variants(7);
     variants[1].batched = false;
     variants[2].direct_max_reduce = tbh::kMaxReduce;
     variants[3].gemm_scatter_bits = -1;
     variants[4].reassociate = false;
     variants[4].split_k = false;
+    variants[5].gemm_tiles = 0;  // 128 x 128 GEMM tiles only
+    variants[6].gemm_tiles = 2;  // 256 x 64 tiles up to N = 128
     for (const auto& opt : variants) {
       tbh::SlicePlan P;
       std::string err;

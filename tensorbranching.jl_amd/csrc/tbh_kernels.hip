@@ -771,6 +771,10 @@ __global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW
 // next unit's registers have landed.  Only units of the same step are
 // prefetched: steps of >= 256 tiles hold nearly all of the work, and a
 // workgroup meets a step boundary at most once per step.
+// Selected by the ew_prefetch tuning (default off): on the 3-regular n=300
+// slices it runs as fast as tbh_ew_kernel alone (144 vs 140 ms of tiled EW
+// per solve) and slower beside the GEMMs of a level (120 VGPRs against 83
+// leave them less room), so staging, not its latency, bounds the step.
 
 __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");

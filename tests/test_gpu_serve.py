@@ -211,3 +211,44 @@ def test_latency_lane(tb, orc, be):
     print(f"big request alone {1e3 * alone_s:.1f} ms; priority slice {1e3 * lat_fast:.1f} ms")
     if alone_s > 0.05:
         assert lat_fast < 0.6 * alone_s, (lat_fast, alone_s)
+
+
+def test_many_producer_threads(tb, orc, be):
+    """tbh_submit and tbh_poll from eight Python threads at once (ctypes
+    releases the GIL inside the calls): every request gets its own value,
+    reported once."""
+    import threading
+    reqs = _requests(tb, orc)
+    results, errors = {}, []
+    lock = threading.Lock()
+
+    def producer(k):
+        try:
+            mine = []
+            for r in range(3):
+                br, dt, want = reqs[(k + r) % len(reqs)]
+                mine.append((srv.submit(br, dt), want))
+            for t, want in mine:
+                while True:
+                    st, v, _, _ = srv.poll(t)
+                    if st == "done":
+                        break
+                    assert st in ("queued", "running"), st
+                    time.sleep(0.001)
+                with lock:
+                    results[t] = (v, want)
+        except Exception as e:  # surfaced below
+            with lock:
+                errors.append(repr(e))
+
+    with tb.Server(be) as srv:
+        threads = [threading.Thread(target=producer, args=(k,)) for k in range(8)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(timeout=120)
+        assert not any(th.is_alive() for th in threads)
+    assert not errors, errors
+    assert len(results) == 24
+    for t, (v, want) in results.items():
+        assert v == want, (t, v, want)

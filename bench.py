@@ -207,11 +207,11 @@ def _fixtures_module():
 
 def serving_mix(tb):
     """The serving configuration's request mix with the oracle's optima: the
-    seeded draw of tests/golden/c5_serving_mix.json, regenerated (digests
+    seeded draw of tests/golden/serving_mix.json, regenerated (digests
     checked) so that every served value is compared with the oracle."""
     mf = _fixtures_module()
-    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "c5_serving_mix.json")))
-    reqs, refused = mf.c5_requests(tb)
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "serving_mix.json")))
+    reqs, refused = mf.serving_draw(tb)
     if refused != fx["refused"] or len(reqs) != len(fx["requests"]):
         raise RuntimeError("serving mix differs from its fixture")
     want = []
@@ -422,7 +422,7 @@ def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out_json
     print(json.dumps(res))
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c3_3regular_n300_sc25.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_3regular_n300_sc25.json")
 
 
 def measured_traffic(kernel):

@@ -8,11 +8,11 @@ DP.  Run from the repository root:  python tests/golden/make_fixtures.py
 
 The full-size fixtures take the oracle minutes to hours of CPU time and are
 written only on request:
-  --c3    the 766 slices of the 3-regular n=300 (seed 2) graph sliced to
+  --n300    the 766 slices of the 3-regular n=300 (seed 2) graph sliced to
           sc_target 25: every per-slice optimum, written with the slice
           set's digest so that a test can regenerate the slices (the host
           slicer is deterministic) and check that they are the same ones;
-  --c5    the per-request optima of a seeded serving mix drawn from the
+  --serving    the per-request optima of a seeded serving mix drawn from the
           serving configuration's ranges (serving_requests(24, seed=5),
           requests up to 2^36 MACs), each with its slice set's digest.
 """
@@ -57,20 +57,20 @@ def sliceset_digest(br):
     return h.hexdigest()
 
 
-C3 = dict(n=300, seed=2, sc_target=25, order_trials=200)
+N300 = dict(n=300, seed=2, sc_target=25, order_trials=200)
 
 
-def c3_slices(tb):
-    g = tb.random_regular_graph(C3["n"], 3, seed=C3["seed"])
-    br = tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=C3["sc_target"],
-                                                          order_trials=C3["order_trials"]),
-                        seed=C3["seed"])
+def n300_slices(tb):
+    g = tb.random_regular_graph(N300["n"], 3, seed=N300["seed"])
+    br = tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=N300["sc_target"],
+                                                          order_trials=N300["order_trials"]),
+                        seed=N300["seed"])
     return g, br
 
 
-def write_c3(tb, oracle, nthreads=0):
+def write_n300(tb, oracle, nthreads=0):
     """Every per-slice optimum of the n=300 workload, by the oracle."""
-    g, br = c3_slices(tb)
+    g, br = n300_slices(tb)
     nthreads = nthreads or os.cpu_count()
     t = time.time()
     res = oracle.contract_slices(br, np.float32, nthreads=nthreads)
@@ -78,30 +78,30 @@ def write_c3(tb, oracle, nthreads=0):
     tcs = [None if b.code is None else round(tb.complexity(b).tc, 4) for b in br]
     json.dump({"workload": "random 3-regular n=300 seed 2, sliced to sc_target 25 "
                            "(order_trials 200), unit weights, Tropical Float32",
-               "params": C3, "slices": len(br), "digest": sliceset_digest(br),
+               "params": N300, "slices": len(br), "digest": sliceset_digest(br),
                "per_slice": res.tolist(), "mis": float(res.max()), "tc": tcs,
                "oracle_wall_s": round(wall, 1), "oracle_threads": nthreads},
-              open(os.path.join(HERE, "c3_n300_sc25.json"), "w"))
-    print(f"c3: {len(br)} slices, MIS {res.max()}, oracle {wall:.0f}s on {nthreads} threads")
+              open(os.path.join(HERE, "regular3_n300_sc25.json"), "w"))
+    print(f"n300: {len(br)} slices, MIS {res.max()}, oracle {wall:.0f}s on {nthreads} threads")
 
 
-C5 = dict(count=24, seed=5, order_trials=50, max_slices=4096, max_log2_macs=36,
+SERVING_DRAW = dict(count=24, seed=5, order_trials=50, max_slices=4096, max_log2_macs=36,
           max_width_gap=16)
 
 
-def c5_requests(tb):
+def serving_draw(tb):
     refused = {}
-    reqs = tb.serving_requests(C5["count"], seed=C5["seed"], order_trials=C5["order_trials"],
-                               max_slices=C5["max_slices"], max_log2_macs=C5["max_log2_macs"],
-                               max_width_gap=C5["max_width_gap"], refused=refused)
+    reqs = tb.serving_requests(SERVING_DRAW["count"], seed=SERVING_DRAW["seed"], order_trials=SERVING_DRAW["order_trials"],
+                               max_slices=SERVING_DRAW["max_slices"], max_log2_macs=SERVING_DRAW["max_log2_macs"],
+                               max_width_gap=SERVING_DRAW["max_width_gap"], refused=refused)
     return reqs, refused
 
 
-def write_c5(tb, oracle):
+def write_serving_draw(tb, oracle):
     """Per-request optima of the seeded serving mix, by the oracle; requests
     with unit weights and n <= 120 are cross-checked against the exact
     branch-and-reduce MIS."""
-    reqs, refused = c5_requests(tb)
+    reqs, refused = serving_draw(tb)
     out = []
     for q in reqs:
         res = oracle.contract_slices(q.branches, q.dtype, nthreads=os.cpu_count())
@@ -112,9 +112,9 @@ def write_c5(tb, oracle):
                     "slices": len(q.branches), "digest": sliceset_digest(q.branches),
                     "log2_macs": round(q.log2_macs, 3),
                     "optimum": float(opt) if q.dtype == np.float32 else int(opt)})
-    json.dump({"mix": tb.SERVING_MIX, "params": C5, "refused": refused, "requests": out},
-              open(os.path.join(HERE, "c5_serving_mix.json"), "w"), indent=0)
-    print(f"c5: {len(out)} requests, refused {refused}")
+    json.dump({"mix": tb.SERVING_MIX, "params": SERVING_DRAW, "refused": refused, "requests": out},
+              open(os.path.join(HERE, "serving_mix.json"), "w"), indent=0)
+    print(f"serving mix: {len(out)} requests, refused {refused}")
 
 
 def main():
@@ -122,7 +122,7 @@ def main():
     from oracle import oracle
     tb = __graft_entry__.package()
 
-    # config 1: 3-regular n=100, seed 0, sliced to sc_target 10, unit weights
+    # 3-regular n=100, seed 0, sliced to sc_target 10, unit weights
     g = tb.random_regular_graph(100, 3, seed=0)
     br = tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=10, order_trials=20), seed=0)
     res = oracle.contract_slices(br, np.float32)
@@ -131,7 +131,7 @@ def main():
     json.dump({"graph": {"nv": g.nv, "edges": g.edges.reshape(-1).tolist()},
                "dtype": "float32", "slices": dump_branches(br),
                "per_slice": res.tolist(), "mis": mis},
-              open(os.path.join(HERE, "c1_regular100.json"), "w"))
+              open(os.path.join(HERE, "regular3_n100_sc10.json"), "w"))
 
     # weighted Int32: 6x6 King's subgraph, filling 0.8, weights in [1, 100]
     g = tb.random_ksg(6, 6, 0.8, seed=3)
@@ -176,12 +176,12 @@ def main():
 
 
 if __name__ == "__main__":
-    if "--c3" in sys.argv or "--c5" in sys.argv:
+    if "--n300" in sys.argv or "--serving" in sys.argv:
         import __graft_entry__
         from oracle import oracle
-        if "--c3" in sys.argv:
-            write_c3(__graft_entry__.package(), oracle)
-        if "--c5" in sys.argv:
-            write_c5(__graft_entry__.package(), oracle)
+        if "--n300" in sys.argv:
+            write_n300(__graft_entry__.package(), oracle)
+        if "--serving" in sys.argv:
+            write_serving_draw(__graft_entry__.package(), oracle)
     else:
         main()

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = ["c1_regular100.json", "ksg6_int32.json", "tree30_f32.json"]
+CASES = ["regular3_n100_sc10.json", "ksg6_int32.json", "tree30_f32.json"]
 
 
 def load_case(tb, name):
@@ -75,8 +75,8 @@ def test_c3_fixture_slices_regenerate(tb, orc):
     """The full-size fixture's slice set is what the host slicer produces
     (same digest), and the oracle reproduces its least-work slices."""
     mf = _make_fixtures()
-    d = json.load(open(os.path.join(HERE, "c3_n300_sc25.json")))
-    g, br = mf.c3_slices(tb)
+    d = json.load(open(os.path.join(HERE, "regular3_n300_sc25.json")))
+    g, br = mf.n300_slices(tb)
     assert len(br) == d["slices"] == 766
     assert mf.sliceset_digest(br) == d["digest"]
     tcs = np.array([np.inf if t is None else t for t in d["tc"]])
@@ -91,8 +91,8 @@ def test_c5_fixture_requests_regenerate(tb, orc):
     (same descriptions, refusals and digests), and the oracle reproduces the
     stored optima of the smaller requests."""
     mf = _make_fixtures()
-    d = json.load(open(os.path.join(HERE, "c5_serving_mix.json")))
-    reqs, refused = mf.c5_requests(tb)
+    d = json.load(open(os.path.join(HERE, "serving_mix.json")))
+    reqs, refused = mf.serving_draw(tb)
     assert refused == d["refused"]
     for q, f in zip(reqs, d["requests"]):
         assert q.desc == f["desc"] and mf.sliceset_digest(q.branches) == f["digest"]

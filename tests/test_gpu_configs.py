@@ -1,8 +1,8 @@
-"""Parity at the full sizes of the benchmark configurations that the oracle
-finishes in seconds: C2 (one unsliced contraction of a 3-regular n=200 graph,
-sc 26), C2's max-plus GEMM shape range, and C4 (40 x 40 King's subgraph,
-filling 0.8, Int32 weights in [1, 100], sc_target 25).  Bit-exact against the
-oracle, per slice."""
+"""Parity at the full sizes of the benchmark workloads that the oracle
+finishes in seconds: one unsliced contraction of a 3-regular n=200 graph
+(sc 26), max-plus GEMMs over that contraction's node shape range
+({2^4 .. 2^12}^3), and a 40 x 40 King's subgraph (filling 0.8, Int32 weights
+in [1, 100], sc_target 25).  Bit-exact against the oracle, per slice."""
 import numpy as np
 import pytest
 
@@ -11,7 +11,7 @@ from conftest import rand_tropical
 pytestmark = pytest.mark.gpu
 
 
-def test_config2_unsliced_n200(tb, orc, backend):
+def test_unsliced_regular3_n200(tb, orc, backend):
     g = tb.random_regular_graph(200, 3, seed=1)
     code = tb.initialize_code(g, order_trials=100, seed=1)
     br = [tb.SlicedBranch(tb.MISProblem(g), code, 0)]
@@ -21,8 +21,8 @@ def test_config2_unsliced_n200(tb, orc, backend):
         assert np.array_equal(backend.contract_slices(br, np.float32, mode), ref), mode
 
 
-def test_config2_gemm_shape_range(backend, orc):
-    """M, N, K drawn from {2^4 .. 2^12} (the node shapes of C2), with the
+def test_gemm_shape_range_2e4_2e12(backend, orc):
+    """M, N, K drawn from {2^4 .. 2^12} (the unsliced n=200 nodes' shapes), with the
     CPU reference's work bounded to 2^27 MACs per product."""
     rng = np.random.default_rng(2)
     done = 0
@@ -36,7 +36,7 @@ def test_config2_gemm_shape_range(backend, orc):
         done += 1
 
 
-def test_config4_ksg40_int32(tb, orc, backend):
+def test_ksg40_int32(tb, orc, backend):
     g = tb.random_ksg(40, 40, 0.8, seed=3)
     w = np.random.default_rng(3).integers(1, 101, g.nv).astype(np.int32)
     br = tb.slice_graph(g, w, tb.ContractionTreeSlicer(sc_target=25, order_trials=50), seed=3)

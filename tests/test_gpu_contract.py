@@ -12,7 +12,7 @@ def _slices(tb, g, sc_target, weights=None, seed=0, trials=20):
 
 
 @pytest.mark.parametrize("mode", ["batched", "plain"])
-def test_config1_unweighted(tb, orc, backend, mode):
+def test_regular3_n100_unweighted(tb, orc, backend, mode):
     """3-regular n=100 seed 0, sliced to sc_target 10: per-slice results equal
     the oracle's and the maximum equals the exact MIS."""
     g = tb.random_regular_graph(100, 3, seed=0)

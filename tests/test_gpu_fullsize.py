@@ -12,8 +12,8 @@ least work are checked bit-exact against the oracle:
 - the maximum lies between a greedy independent set and n/2 (a bound that
   holds for every regular graph).
 Every one of the 766 per-slice optima is also pinned: the oracle computed
-them once (tests/golden/c3_n300_sc25.json, about 450 s on 8 CPU threads,
-written by make_fixtures.py --c3 with the slice set's digest), and the test
+them once (tests/golden/regular3_n300_sc25.json, about 450 s on 8 CPU threads,
+written by make_fixtures.py --n300 with the slice set's digest), and the test
 regenerates the slice set, checks that its digest matches, and compares the
 GPU's results with the stored optima bit for bit."""
 import importlib.util
@@ -39,7 +39,7 @@ N, SEED = 300, 2
 
 
 @pytest.fixture(scope="module")
-def c3(tb):
+def n300(tb):
     g = tb.random_regular_graph(N, 3, seed=SEED)
     br = tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=25, order_trials=200),
                         seed=SEED)
@@ -66,8 +66,8 @@ def _least_work(tb, br, k):
     return np.argsort(tcs, kind="stable")[:k]
 
 
-def test_full_size_solve_properties(tb, orc, backend, c3):
-    g, br = c3
+def test_full_size_solve_properties(tb, orc, backend, n300):
+    g, br = n300
     assert len(br) > 500
     assert max(tb.sc(b) for b in br if b.code is not None) <= 25
     batched = backend.contract_slices(br, np.float32, "batched")
@@ -89,8 +89,8 @@ def test_full_size_solve_properties(tb, orc, backend, c3):
     assert np.array_equal(batched[idx], ref)
 
 
-def test_full_size_slicing_invariant(tb, orc, backend, c3):
-    g, br25 = c3
+def test_full_size_slicing_invariant(tb, orc, backend, n300):
+    g, br25 = n300
     br22 = tb.slice_graph(g, None, tb.ContractionTreeSlicer(sc_target=22, order_trials=200),
                           seed=SEED)
     assert len(br22) > len(br25)
@@ -103,12 +103,12 @@ def test_full_size_slicing_invariant(tb, orc, backend, c3):
 
 
 @pytest.mark.parametrize("mode", ["batched", "plain"])
-def test_full_size_every_slice_against_oracle(tb, backend, c3, mode):
+def test_full_size_every_slice_against_oracle(tb, backend, n300, mode):
     """All 766 slices of the n=300 workload, large ones included (2^25-entry
     tensors, split-K GEMMs, LDS-staged GEMM outputs, outer-reduced EW
     steps), bit-exact against the oracle's stored per-slice optima."""
-    g, br = c3
-    fx = json.load(open(os.path.join(GOLDEN, "c3_n300_sc25.json")))
+    g, br = n300
+    fx = json.load(open(os.path.join(GOLDEN, "regular3_n300_sc25.json")))
     assert fx["params"] == {"n": N, "seed": SEED, "sc_target": 25, "order_trials": 200}
     assert len(br) == fx["slices"]
     assert _fixtures_module().sliceset_digest(br) == fx["digest"]

@@ -6,8 +6,8 @@ continuous-batching scheduler over a 200 GB pool.  Draws beyond the host
 slicer's reach are refused by serving_requests (the counts are part of the
 fixture), as are requests above 2^36 MACs, which keeps the oracle's work for
 the fixture to about a minute.  Every served value is checked against the
-oracle's optimum stored in tests/golden/c5_serving_mix.json (written by
-make_fixtures.py --c5 with each request's slice-set digest), alone and in
+oracle's optimum stored in tests/golden/serving_mix.json (written by
+make_fixtures.py --serving with each request's slice-set digest), alone and in
 the mixed stream (maximum over slices, src/dynamic_ob.jl:24-27)."""
 import importlib.util
 import json
@@ -31,13 +31,13 @@ def _fixtures_module():
 
 
 @pytest.fixture(scope="module")
-def c5(tb):
+def mix(tb):
     mf = _fixtures_module()
-    fx = json.load(open(os.path.join(GOLDEN, "c5_serving_mix.json")))
-    assert fx["params"] == mf.C5
+    fx = json.load(open(os.path.join(GOLDEN, "serving_mix.json")))
+    assert fx["params"] == mf.SERVING_DRAW
     t = time.time()
-    reqs, refused = mf.c5_requests(tb)
-    print(f"[c5] {len(reqs)} requests drawn in {time.time() - t:.1f}s, refused {refused}",
+    reqs, refused = mf.serving_draw(tb)
+    print(f"[mix] {len(reqs)} requests drawn in {time.time() - t:.1f}s, refused {refused}",
           flush=True)
     assert refused == fx["refused"]
     assert len(reqs) == len(fx["requests"])
@@ -50,8 +50,8 @@ def c5(tb):
 
 
 @pytest.mark.timeout(900)
-def test_c5_mix_covers_the_ranges(c5):
-    reqs, _ = c5
+def test_serving_mix_covers_the_ranges(mix):
+    reqs, _ = mix
     kinds = {q.kind for q in reqs}
     dtypes = {np.dtype(q.dtype) for q in reqs}
     assert kinds == {"3-regular", "ER"} and dtypes == {np.dtype(np.float32), np.dtype(np.int32)}
@@ -61,12 +61,12 @@ def test_c5_mix_covers_the_ranges(c5):
 
 
 @pytest.mark.timeout(900)
-def test_c5_served_alone_and_poisson_against_oracle(tb, c5):
+def test_serving_mix_alone_and_poisson_against_oracle(tb, mix):
     """Each request served alone, then the whole mix twice over as Poisson
     arrivals (at about twice the rate the requests are served alone, so
     rounds mix requests), over a 200 GB pool: every value equals the
     oracle's, in its own element type."""
-    reqs, want = c5
+    reqs, want = mix
     be = tb.HIPBackend(0, pool_bytes=200 << 30)
     try:
         with tb.Server(be) as srv:
@@ -102,7 +102,7 @@ def test_c5_served_alone_and_poisson_against_oracle(tb, c5):
                 lat.append(td - ts)
             st = srv.stats()
         lat = np.sort(lat)
-        print(f"[c5] alone {1e3 * t_alone:.1f} ms for {len(reqs)} requests; Poisson "
+        print(f"[mix] alone {1e3 * t_alone:.1f} ms for {len(reqs)} requests; Poisson "
               f"{rate:.0f} req/s x {len(order)}: p50 {1e3 * np.percentile(lat, 50):.2f} ms, "
               f"p99 {1e3 * np.percentile(lat, 99):.2f} ms, rounds {st['rounds']}", flush=True)
         assert st["failed_requests"] == 0 and st["retried_rounds"] == 0

@@ -181,7 +181,7 @@ def test_order_invariance_and_slicing_invariant(tb, orc):
     assert orc.contract_slices(br, np.float32).max() == orc.mis_exact(g)
 
 
-def test_config1_mis(tb, orc):
+def test_regular3_n100_mis(tb, orc):
     """3-regular n=100 seed 0 sliced to sc_target 10: max over slices (+r)
     equals the exact MIS (README.md:38-43 shows 44 for an unseeded graph)."""
     g = tb.random_regular_graph(100, 3, seed=0)

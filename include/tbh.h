@@ -277,8 +277,10 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode,
  * persistent grid; more: later workgroups start as earlier ones finish,
  * which evens out uneven tiles); gemm_tiles: GEMM tile shapes (0: 128 x 128
  * only; 1: 256 x 64 / 256 x 32 tiles for outputs of M >= 256 rows and N <= 64
- * / 32 columns; 2: 256 x 64 up to N = 128 too).  Results never depend on
- * them. */
+ * / 32 columns; 2: 256 x 64 up to N = 128 too); chain_cost > 0: a chain
+ * rotation is taken only when it lowers the estimated cost of the two steps
+ * it replaces, bytes moved + MACs * chain_cost / 64, in place of the
+ * chain_reduce cap (0: the caps alone).  Results never depend on them. */
 typedef struct {
   int32_t chain_big, chain_side, chain_reduce;
   int32_t split_tiles_bits, split_min_k_bits;
@@ -290,6 +292,7 @@ typedef struct {
   int32_t ew_prefetch;
   int32_t ew_grid_mult, gemm_grid_mult;
   int32_t gemm_tiles;
+  int32_t chain_cost;
 } tbh_tuning;
 int tbh_get_tuning(tbh_ctx* ctx, tbh_tuning* out);
 int tbh_set_tuning(tbh_ctx* ctx, const tbh_tuning* t);

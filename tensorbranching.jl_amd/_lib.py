@@ -56,7 +56,7 @@ class TbhTuning(C.Structure):
                 ("direct_max_reduce", C.c_int32), ("concurrent", C.c_int32),
                 ("first_batch", C.c_double), ("ew_prefetch", C.c_int32),
                 ("ew_grid_mult", C.c_int32), ("gemm_grid_mult", C.c_int32),
-                ("gemm_tiles", C.c_int32)]
+                ("gemm_tiles", C.c_int32), ("chain_cost", C.c_int32)]
 
 
 class TbhStepInfo(C.Structure):

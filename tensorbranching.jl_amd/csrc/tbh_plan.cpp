@@ -656,13 +656,32 @@ void reassociate(int32_t nl, int32_t nv, std::vector<int32_t>& tree,
       int Yp = rk[lc[Y]] >= rk[rc[Y]] ? lc[Y] : rc[Y];
       int Sp = Yp == lc[Y] ? rc[Y] : lc[Y];
       if (rk[Yp] < opt.chain_big || rk[Sp] > side) break;
-      int rf = 0, nred = 0;
+      int rf = 0, nred = 0, u_ys = 0, u_x = 0, u_f = 0, u_new = 0;
       for (int i = 0; i < W; i++) {
         F[i] = (Lp(Sp)[i] | Lp(S)[i]) & (Lp(Yp)[i] | Lp(x)[i]);
         rf += __builtin_popcountll(F[i]);
         nred += __builtin_popcountll((Lp(Yp)[i] | F[i]) & ~Lp(x)[i]);
+        u_ys += __builtin_popcountll(Lp(Yp)[i] | Lp(Sp)[i]);
+        u_x += __builtin_popcountll(Lp(Y)[i] | Lp(S)[i]);
+        u_f += __builtin_popcountll(Lp(Sp)[i] | Lp(S)[i]);
+        u_new += __builtin_popcountll(Lp(Yp)[i] | F[i]);
       }
-      if (rf > side || nred > opt.chain_reduce) break;
+      if (rf > side) break;
+      if (opt.chain_cost > 0) {
+        // old: Y = Y' (x) S' then x = Y (x) S; new: F = S' (x) S then
+        // x = Y' (x) F.  Reads of Y', S', S and the write of x are common;
+        // the old pair writes and reads Y, the new one F.
+        // (a weight per step kind -- EW MACs dearer than GEMM MACs -- chose
+        // worse rotations on the 3-regular n=300 slices than one weight)
+        const double w = opt.chain_cost / 64.0;
+        const double c_old = 8 * std::ldexp(1.0, rk[Y]) +
+                             w * (std::ldexp(1.0, u_ys) + std::ldexp(1.0, u_x));
+        const double c_new = 8 * std::ldexp(1.0, rf) +
+                             w * (std::ldexp(1.0, u_f) + std::ldexp(1.0, u_new));
+        if (c_new >= c_old) break;
+      } else if (nred > opt.chain_reduce) {
+        break;
+      }
       // node Y becomes F = (S', S); x = (Y', F)
       lc[Y] = Sp;
       rc[Y] = S;

@@ -85,6 +85,12 @@ struct PlanOptions {
   // tensor <= chain_side, at most chain_reduce labels reduced per step
   bool reassociate = true;
   int chain_big = 17, chain_side = 16, chain_reduce = 7;
+  // chain_cost > 0: a rotation is also taken only when it lowers the two
+  // steps' estimated cost, bytes moved + MACs * chain_cost / 64 (chain_cost
+  // / 64 bytes per MAC), and the chain_reduce cap is lifted; 0: caps only.
+  // On the 3-regular n=300 slices (sc 25) 12 measured 272 ms per solve
+  // against 282 ms with the caps alone (8: 277, 16: 274, 24: 279)
+  int chain_cost = 12;
   // a GEMM node writes its output straight in the layout its consumer asks
   // for (instead of a permute step after it) when the output's lowest
   // gemm_scatter_bits positions (at least 2) are row or column bits of one

@@ -254,6 +254,7 @@ tbh_tuning default_tuning() {
     d.ew_grid_mult = 16;
     d.gemm_grid_mult = 16;
     d.gemm_tiles = 1;
+    d.chain_cost = o.chain_cost;
     if (const char* e = std::getenv("TBH_CHAIN"))
       std::sscanf(e, "%d,%d,%d", &d.chain_big, &d.chain_side, &d.chain_reduce);
     if (const char* e = std::getenv("TBH_SPLITK"))
@@ -274,15 +275,16 @@ tbh_tuning default_tuning() {
       d.gemm_grid_mult = std::min(64, std::max(1, std::atoi(e)));
     if (const char* e = std::getenv("TBH_GEMM_TILES"))
       d.gemm_tiles = std::min(2, std::max(0, std::atoi(e)));
+    if (const char* e = std::getenv("TBH_CHAIN_COST"))
+      d.chain_cost = std::min(4096, std::max(0, std::atoi(e)));
     return d;
   }();
   return t;
 }
 
-PlanOptions plan_options(const tbh_ctx* ctx, bool batched) {
+PlanOptions plan_options(const tbh_tuning& t, bool batched) {
   PlanOptions o;
   o.batched = batched;
-  const tbh_tuning& t = ctx->tune;
   o.chain_big = t.chain_big;
   o.chain_side = t.chain_side;
   o.chain_reduce = t.chain_reduce;
@@ -295,7 +297,12 @@ PlanOptions plan_options(const tbh_ctx* ctx, bool batched) {
   o.gemm_scatter_bits = scatter_bits;
   o.direct_max_reduce = t.direct_max_reduce;
   o.gemm_tiles = t.gemm_tiles;
+  o.chain_cost = t.chain_cost;
   return o;
+}
+
+PlanOptions plan_options(const tbh_ctx* ctx, bool batched) {
+  return plan_options(ctx->tune, batched);
 }
 
 void WorkerPool::start(int threads) {
@@ -1603,8 +1610,7 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode, tbh_s
   int rc = convert_slice(*slice, dtype, false, &o);
   if (rc) return rc;
   SlicePlan P;
-  PlanOptions opt;
-  opt.batched = mode == TBH_BATCHED;
+  const PlanOptions opt = plan_options(default_tuning(), mode == TBH_BATCHED);
   std::string err;
   if (plan_slice(o.in, opt, &P, &err)) {
     set_error(err);
@@ -1659,7 +1665,7 @@ static int check_tuning(const tbh_tuning& t) {
       t.concurrent > 1 ||
       !(t.first_batch > 0.0 && t.first_batch <= 1.0) || t.ew_prefetch < 0 || t.ew_prefetch > 1 ||
       t.ew_grid_mult < 1 || t.ew_grid_mult > 64 || t.gemm_grid_mult < 1 || t.gemm_grid_mult > 64 ||
-      t.gemm_tiles < 0 || t.gemm_tiles > 2) {
+      t.gemm_tiles < 0 || t.gemm_tiles > 2 || t.chain_cost < 0 || t.chain_cost > 4096) {
     set_error("tuning parameter out of range");
     return TBH_EINVAL;
   }
@@ -1794,7 +1800,8 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
       {{0, 1}, nullptr, &tbh_tuning::ew_prefetch},
       {{1, 4, 16, 64}, nullptr, &tbh_tuning::ew_grid_mult},
       {{1, 4, 16, 64}, nullptr, &tbh_tuning::gemm_grid_mult},
-      {{0, 1, 2}, nullptr, &tbh_tuning::gemm_tiles}};
+      {{0, 1, 2}, nullptr, &tbh_tuning::gemm_tiles},
+      {{0, 8, 12, 16, 24}, nullptr, &tbh_tuning::chain_cost}};
   // chain re-association regroups a chain's sums: for Float32 slices whose
   // sums are not all exact (non-integer weights or r, or sums reaching
   // 2^24) a result may move by a rounding step, so the chain parameters are
@@ -1802,7 +1809,7 @@ int tbh_autotune(tbh_ctx* ctx, tbh_dtype dtype, const tbh_slice* slices, size_t 
   const bool exact = exact_sums(dtype, slices, n);
   auto searched = [&](const Param& p) {
     return exact || (p.i != &tbh_tuning::chain_big && p.i != &tbh_tuning::chain_side &&
-                     p.i != &tbh_tuning::chain_reduce);
+                     p.i != &tbh_tuning::chain_reduce && p.i != &tbh_tuning::chain_cost);
   };
   auto get = [](const tbh_tuning& t, const Param& p) { return p.d ? t.*p.d : (double)(t.*p.i); };
   auto set = [](tbh_tuning& t, const Param& p, double v) {

@@ -252,11 +252,13 @@ int execute_plain(tbh_ctx* ctx, int dtype, const SliceOwned& slice, const SliceP
 size_t plain_device_bytes(const SliceOwned& slice, const SlicePlan& plan);
 int plan_all(tbh_ctx* ctx, const std::vector<const SliceOwned*>& slices, bool batched,
              std::vector<SlicePlan>* plans);
-// the context's tuning as planner options
+// a tuning (the context's) as planner options
+PlanOptions plan_options(const tbh_tuning& t, bool batched);
 PlanOptions plan_options(const tbh_ctx* ctx, bool batched);
 // default tuning (environment overrides TBH_CHAIN="big,side,reduce",
-// TBH_SPLITK="tile_bits,min_k_bits", TBH_EW_CHUNK, TBH_DIRECT_CHUNK,
-// TBH_FIRST_BATCH, read once)
+// TBH_SPLITK="tile_bits,min_k_bits", TBH_CHAIN_COST, TBH_EW_CHUNK,
+// TBH_DIRECT_CHUNK, TBH_FIRST_BATCH and the others listed where it is
+// defined, read once)
 tbh_tuning default_tuning();
 
 }  // namespace tbh

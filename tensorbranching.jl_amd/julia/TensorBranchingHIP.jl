@@ -188,6 +188,7 @@ mutable struct TbhTuning
     ew_prefetch::Int32
     ew_grid_mult::Int32; gemm_grid_mult::Int32
     gemm_tiles::Int32
+    chain_cost::Int32
     TbhTuning() = new()
 end
 

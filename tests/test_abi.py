@@ -172,7 +172,7 @@ def test_serving_entry_points_reject_null_context(tb):
 def test_tuning_struct_and_null_context(tb):
     from tensorbranching_jl_amd import _lib
     lib = _lib.tbh()
-    assert C.sizeof(_lib.TbhTuning) == 64  # 10 int32, a double, 4 int32
+    assert C.sizeof(_lib.TbhTuning) == 72  # 10 int32, a double, 5 int32, padding to 8
     t = _lib.TbhTuning()
     assert lib.tbh_get_tuning(None, C.byref(t)) == _lib.TBH_EINVAL
     assert lib.tbh_set_tuning(None, C.byref(t)) == _lib.TBH_EINVAL

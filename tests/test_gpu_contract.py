@@ -135,13 +135,15 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
     # direct wave tiles with reductions (every eligible step direct), no
     # direct steps at all, every launch on one stream (GEMMs not beside the
     # EW steps of their level), tiled EW steps prefetched into registers
-    # (alone and with every EW step tiled), and persistent grids
+    # (alone and with every EW step tiled), persistent grids, both GEMM tile
+    # rules besides the default, and chain rotations under the rank caps
+    # alone (no cost test)
     t0 = backend.tuning()
     try:
         for kw in ({"direct_max_reduce": 12}, {"direct_max_reduce": -1}, {"concurrent": 0},
                    {"ew_prefetch": 1}, {"ew_prefetch": 1, "direct_max_reduce": -1},
                    {"ew_grid_mult": 1, "gemm_grid_mult": 1}, {"gemm_tiles": 0},
-                   {"gemm_tiles": 2}):
+                   {"gemm_tiles": 2}, {"chain_cost": 0}):
             backend.set_tuning(**dict(t0, **kw))
             assert np.array_equal(backend.contract_slices(br, dtype, "batched"), ref), kw
     finally:
@@ -220,7 +222,7 @@ def test_autotune_keeps_results(tb, orc):
         assert np.array_equal(be.contract_slices(br, np.float32), orc.contract_slices(br, np.float32))
         for bad in ({"ew_chunk": 0}, {"ew_per_cu": 9}, {"direct_max_reduce": 13},
                     {"concurrent": 2}, {"first_batch": 0.0}, {"gemm_tiles": 3},
-                    {"ew_grid_mult": 0}, {"ew_prefetch": 2}):
+                    {"ew_grid_mult": 0}, {"ew_prefetch": 2}, {"chain_cost": -1}):
             with pytest.raises(tb.TbhError):
                 be.set_tuning(**bad)
         be.set_tuning(**t0)
@@ -272,8 +274,8 @@ def test_autotune_fractional_float32_weights(tb, orc):
         be.set_tuning(**dict(t0, chain_big=30))
         best, ms = be.autotune(br, np.float32, max_trials=8)
         assert ms > 0 and best == be.tuning()
-        assert (best["chain_big"], best["chain_side"], best["chain_reduce"]) == \
-            (30, t0["chain_side"], t0["chain_reduce"])
+        assert (best["chain_big"], best["chain_side"], best["chain_reduce"], best["chain_cost"]) == \
+            (30, t0["chain_side"], t0["chain_reduce"], t0["chain_cost"])
         assert np.array_equal(be.contract_slices(br, np.float32, "batched"), ref)
     finally:
         be.close()

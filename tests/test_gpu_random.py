@@ -3,7 +3,8 @@ Erdos-Renyi, King's subgraph), size, weight type (unit, integer Float32,
 Int32), slicing target (up to six below the graph's unsliced sc) and a
 tuning variant (defaults, every eligible EW step direct, none direct, one
 stream, no chain re-association, tiled EW prefetched into registers, persistent
-grids, 128 x 128 GEMM tiles only, wider tall-skinny GEMM tiles) drawn per case.  Each case's per-slice
+grids, 128 x 128 GEMM tiles only, wider tall-skinny GEMM tiles, chain rotations
+under the rank caps alone) drawn per case.  Each case's per-slice
 optima must equal the oracle's, bit for bit, in both executors.  Cases whose
 unsliced sc exceeds 28, whose slices exceed 2^28 work or 4000 in number are
 skipped (deterministically, from the seed), so the oracle stays within
@@ -33,7 +34,7 @@ def _case(tb, rng):
     wt = str(rng.choice(["none", "f32", "i32"]))
     dtype = np.int32 if wt == "i32" else np.float32
     w = None if wt == "none" else rng.integers(1, 101, g.nv).astype(dtype)
-    variant = int(rng.integers(9))
+    variant = int(rng.integers(len(VARIANTS)))
     drop = int(rng.integers(0, 7))
     nat = tb.sc(tb.SlicedBranch(tb.MISProblem(g), tb.initialize_code(g, order_trials=5, seed=seed), 0))
     desc = f"{fam} nv={g.nv} seed={seed} weights={wt} variant={variant}"
@@ -52,7 +53,7 @@ def _case(tb, rng):
 
 VARIANTS = [{}, {"direct_max_reduce": 12}, {"direct_max_reduce": -1}, {"concurrent": 0},
             {"chain_big": 30}, {"ew_prefetch": 1}, {"ew_grid_mult": 1, "gemm_grid_mult": 1},
-            {"gemm_tiles": 0}, {"gemm_tiles": 2}]
+            {"gemm_tiles": 0}, {"gemm_tiles": 2}, {"chain_cost": 0}]
 
 
 @pytest.mark.parametrize("group", range(GROUPS))

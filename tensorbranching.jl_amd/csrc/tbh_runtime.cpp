@@ -1252,6 +1252,8 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
   ctx->pool.init(pool_bytes);
   ctx->stats.pool_bytes = ctx->pool.capacity();
   ctx->plan_threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (const char* e = std::getenv("TBH_PLAN_THREADS"))
+    ctx->plan_threads = std::min(64, std::max(1, std::atoi(e)));
   ctx->workers.start(2 * ctx->plan_threads);
   ctx->tune = default_tuning();
   resolve_scratch(device);  // for tbh_contract_slices_noalloc (device is current)

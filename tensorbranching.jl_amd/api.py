@@ -127,6 +127,17 @@ class CompressedEinsum:
     def nleaves(self):
         return len(self.ix_ptr) - 1
 
+    def _c_args(self):
+        """(nleaves, ix_ptr, ix, tree) for a tbh_slice, the arrays as
+        addresses; kept while the three arrays are the same objects."""
+        c = self.__dict__.get("_c")
+        if c is None or c[0] is not self.ix_ptr or c[1] is not self.ix or c[2] is not self.tree:
+            c = (self.ix_ptr, self.ix, self.tree,
+                 (len(self.ix_ptr) - 1, self.ix_ptr.ctypes.data, self.ix.ctypes.data,
+                  self.tree.ctypes.data if len(self.tree) else None))
+            self._c = c
+        return c[3]
+
     @property
     def ixs(self):
         return [self.ix[self.ix_ptr[i]:self.ix_ptr[i + 1]].tolist() for i in range(self.nleaves)]
@@ -235,10 +246,7 @@ class _Marshal:
                 s.nleaves = 0
                 continue
             c = b.code
-            s.nleaves = c.nleaves
-            s.ix_ptr = c.ix_ptr.ctypes.data
-            s.ix = c.ix.ctypes.data
-            s.tree = c.tree.ctypes.data if len(c.tree) else None
+            s.nleaves, s.ix_ptr, s.ix, s.tree = c._c_args()
             self.keep.append(c)
         self.n = n
 

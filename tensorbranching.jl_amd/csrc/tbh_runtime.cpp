@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <system_error>
 
 namespace tbh {
 
@@ -602,7 +603,10 @@ size_t region_bytes(size_t largest) {
 
 // Batches run one after another on the stream through one arena region
 // (stream order keeps a batch's kernels behind the previous batch's).  A
-// batch is packed in two passes on the host while the previous batch runs:
+// batch's launches are issued by a thread of its own (a launch call blocks
+// while the device's queue is deep), and its descriptors go up on a copy
+// stream, so the next batch is packed and uploaded while this one is still
+// being issued and run.  A batch is packed in two passes on the host:
 // pass 1 (serial, integer work only) counts the steps of every (kernel kind,
 // level), gives each slice its first index in each and builds the tile
 // prefixes; pass 2 (host threads, one slice at a time) writes every
@@ -630,6 +634,9 @@ int BatchRunner::open() {
 }
 
 void BatchRunner::close() {
+  if (issuer_.joinable()) issuer_.join();
+  issue_err_ = hipSuccess;  // a failed batch is released below
+  issue_msg_.clear();
   if (prev_.on) {
     // an unfinished batch: wait for it without reporting (error paths)
     sync_all();
@@ -648,8 +655,20 @@ void BatchRunner::close() {
 // Waits for both streams (error paths: no kernel may still use the arena
 // or the descriptors when they are released).
 void BatchRunner::sync_all() {
+  (void)hipStreamSynchronize(ctx_->copy_stream);
   (void)hipStreamSynchronize(ctx_->stream);
   if (ctx_->stream2) (void)hipStreamSynchronize(ctx_->stream2);
+}
+
+// Waits until every launch of the newest batch is issued; reports a failed
+// launch (the batch stays pending, for close() to wait for and release).
+int BatchRunner::join_issue() {
+  if (issuer_.joinable()) issuer_.join();
+  if (issue_err_ != hipSuccess) {
+    set_error(issue_msg_);
+    return TBH_EDEVICE;
+  }
+  return 0;
 }
 
 // Waits for the batch in flight, releases its descriptors and copies its
@@ -671,7 +690,151 @@ int BatchRunner::finish(void** done_tag) {
   return 0;
 }
 
-int BatchRunner::flush(void** done_tag) { return finish(done_tag); }
+int BatchRunner::flush(void** done_tag) {
+  if (done_tag) *done_tag = nullptr;
+  const int rc = join_issue();
+  return rc ? rc : finish(done_tag);
+}
+
+// One batch's launch sequence, issued by the batch's own thread: the wait
+// for its descriptor upload, the leaves, one launch per kernel kind and
+// level, the epilogue and the copy of the results back, then its done event.
+struct IssueJob {
+  int dtype = 0;
+  bool prof = false, per_step = false;
+  int bidx = 0, H = 0;
+  size_t nb = 0;
+  int64_t nleaves = 0;
+  size_t o_leaves = 0, o_epi = 0, o_res = 0;
+  char* dbase = nullptr;     // the batch's descriptors on the device
+  const char* hb = nullptr;  // ... and in the pinned staging buffer (step tags)
+  hipEvent_t up = nullptr, done = nullptr;
+  std::vector<int32_t> cnt;
+  std::vector<std::vector<int64_t>> pfx;
+  std::vector<size_t> o_desc, o_pfx, o_one;
+  std::vector<double> lvl_bytes, lvl_macs;
+  std::vector<std::vector<std::pair<double, double>>> step_work;
+};
+
+namespace {
+
+hipError_t issue_batch(tbh_ctx* ctx, IssueJob& j) {
+  hipError_t e = hipSetDevice(ctx->device);
+  auto t_issue = std::chrono::steady_clock::now();
+  const int H = j.H, dtype = j.dtype;
+  const bool prof = j.prof;
+  auto slot = [H](int k, int L) { return k * (H + 1) + L; };
+  const auto& cnt = j.cnt;
+  const auto& pfx = j.pfx;
+  char* dbase = j.dbase;
+  if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, j.up, 0);
+  if (e == hipSuccess)
+    e = launch_leaves(dtype, (const LeafDesc*)(dbase + j.o_leaves), j.nleaves, ctx->stream);
+  ctx->stats.launches += j.nleaves ? 1 : 0;
+  hipEvent_t ev = nullptr;
+  // one launch per kernel kind and level (direct EW, tiled EW, GEMM); in
+  // the per-step profiling mode every step alone, tagged with its shape
+  // GEMM launches on a second stream beside a level's EW launches (not
+  // while profiling): a level's steps are independent, so the VALU-bound
+  // GEMM tiles and the memory-bound EW tiles can share the CUs.  Each
+  // stream waits for the other's work of earlier levels (events recorded
+  // lazily, only when the other stream launched since the last sync).
+  const bool conc = ctx->tune.concurrent != 0 && !prof;
+  hipStream_t sk[2] = {ctx->stream, conc ? ctx->stream2 : ctx->stream};
+  auto& evpool = ctx->lvl_events[j.bidx & 1];
+  size_t evn = 0;
+  // dirty[s]: stream s launched work the other stream has not waited for
+  // (the upload wait and the leaves are stream-0 work)
+  bool dirty[2] = {true, false};
+  auto sync_to = [&](int s) {  // stream s waits for the other stream's work so far
+    const int o = 1 - s;
+    if (!conc || !dirty[o]) return hipSuccess;
+    if (evn == evpool.size()) {
+      hipEvent_t x;
+      hipError_t r = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+      if (r != hipSuccess) return r;
+      evpool.push_back(x);
+    }
+    hipEvent_t x = evpool[evn++];
+    hipError_t r = hipEventRecord(x, sk[o]);
+    if (r == hipSuccess) r = hipStreamWaitEvent(sk[s], x, 0);
+    dirty[o] = false;
+    return r;
+  };
+  auto launch = [&](int k, const char* descs, const int64_t* prefix, int nops, int64_t total) {
+    if (k == 0)
+      return launch_ew_direct(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
+                              direct_grid(ctx, total, direct_chunk(ctx, total)),
+                              direct_chunk(ctx, total), sk[0]);
+    if (k == 1)
+      return (ctx->tune.ew_prefetch ? launch_ew_pf : launch_ew)(
+          dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
+                       ew_grid(ctx, dtype, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
+                       sk[0]);
+    return launch_gemm(dtype, true, k - 2, reinterpret_cast<const GemmDesc*>(descs), prefix, nops,
+                       total, gemm_grid(ctx, total, dtype, true, k - 2), sk[1]);
+  };
+  for (int L = 1; L <= H && e == hipSuccess; L++) {
+    if (conc) {
+      // a stream launching work of this level first waits for the other
+      // stream's work of earlier levels (both syncs before any launch)
+      bool has[2] = {false, false};
+      for (int k = 0; k < kKinds; k++)
+        if (cnt[slot(k, L)]) has[k >= 2 ? 1 : 0] = true;
+      if (has[0] && e == hipSuccess) e = sync_to(0);
+      if (has[1] && e == hipSuccess) e = sync_to(1);
+      if (has[0]) dirty[0] = true;
+      if (has[1]) dirty[1] = true;
+    }
+    for (int k = 0; k < kKinds && e == hipSuccess; k++) {
+      const int w = slot(k, L);
+      if (!cnt[w]) continue;
+      const int64_t total = pfx[w].back();
+      if (j.per_step) {
+        for (int32_t q = 0; q < cnt[w] && e == hipSuccess; q++) {
+          char tag_s[160];
+          if (k >= 2) {
+            const GemmDesc& d = reinterpret_cast<const GemmDesc*>(j.hb + j.o_desc[w])[q];
+            std::snprintf(tag_s, sizeof tag_s, "M=%d N=%d K=%d batch=%d", d.M, d.N, d.K, d.batch);
+          } else {
+            const EwDesc& d = reinterpret_cast<const EwDesc*>(j.hb + j.o_desc[w])[q];
+            std::snprintf(tag_s, sizeof tag_s, "nT=%d nO=%d nR=%d nRo=%d nSA=%d nSB=%d op=%d",
+                          d.nT, d.nO, d.nR, d.nRo, d.nSA, d.nSB, d.mlay);
+          }
+          const int64_t tiles = pfx[w][q + 1] - pfx[w][q];
+          ev = ctx->prof.begin(ctx->stream);
+          e = launch(k, dbase + j.o_desc[w] + q * desc_size(k),
+                     reinterpret_cast<const int64_t*>(dbase + j.o_one[w]) + 2 * q, 1, tiles);
+          ctx->prof.end(ctx->stream, ev, stat_kind(k), L, 1, j.step_work[w][q].first,
+                        j.step_work[w][q].second, tag_s);
+          ctx->stats.launches++;
+        }
+        continue;
+      }
+      if (prof) ev = ctx->prof.begin(ctx->stream);
+      e = launch(k, dbase + j.o_desc[w], reinterpret_cast<const int64_t*>(dbase + j.o_pfx[w]),
+                 cnt[w], total);
+      if (prof)
+        ctx->prof.end(ctx->stream, ev, stat_kind(k), L, cnt[w], j.lvl_bytes[w], j.lvl_macs[w]);
+      ctx->stats.launches++;
+    }
+    ctx->stats.waves++;
+  }
+  if (e == hipSuccess && conc) e = sync_to(0);  // the last GEMMs before the epilogue
+  if (e == hipSuccess) {
+    e = launch_epilogue(dtype, (const EpiDesc*)(dbase + j.o_epi), (int64_t)j.nb, dbase + j.o_res,
+                        ctx->stream);
+    ctx->stats.launches++;
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ctx->results_pin[j.bidx & 1].host, dbase + j.o_res, 4 * j.nb,
+                       hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && j.done) e = hipEventRecord(j.done, ctx->stream);
+  ctx->stats.host_ms[2] += ms_since(t_issue);
+  return e;
+}
+
+}  // namespace
 
 int BatchRunner::enqueue(int dtype, const std::vector<const SliceOwned*>& slices,
                          const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* dst,
@@ -805,134 +968,85 @@ int BatchRunner::enqueue(int dtype, const std::vector<const SliceOwned*>& slices
   // device descriptors: if the reserve cannot hold two batches' worth,
   // wait for the batch in flight first
   int64_t doff = ctx->pool.alloc(bytes);
+  void* early_tag = nullptr;
   if (doff < 0) {
-    if ((rc = finish(done_tag))) return rc;
+    if ((rc = join_issue())) return rc;
+    if ((rc = finish(&early_tag))) return rc;
     doff = ctx->pool.alloc(bytes);
   }
   if (doff < 0) {
     set_error("pool exhausted (descriptors)");
     return TBH_ENOMEM;
   }
-  auto t_issue = std::chrono::steady_clock::now();
+  // the upload goes on the copy stream, beside the batch in flight (its
+  // descriptors are a pool block of their own); the issuing thread makes the
+  // compute stream wait for it
   char* dbase = ctx->base + doff;
-  hipError_t e = hipMemcpyAsync(dbase, hb, bytes, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess)
-    e = launch_leaves(dtype, (const LeafDesc*)(dbase + o_leaves), nleaves, ctx->stream);
-  ctx->stats.launches += nleaves ? 1 : 0;
-  hipEvent_t ev = nullptr;
-  // one launch per kernel kind and level (direct EW, tiled EW, GEMM); in
-  // the per-step profiling mode every step alone, tagged with its shape
-  // GEMM launches on a second stream beside a level's EW launches (not
-  // while profiling): a level's steps are independent, so the VALU-bound
-  // GEMM tiles and the memory-bound EW tiles can share the CUs.  Each
-  // stream waits for the other's work of earlier levels (events recorded
-  // lazily, only when the other stream launched since the last sync).
-  const bool conc = ctx->tune.concurrent != 0 && !prof;
-  hipStream_t sk[2] = {ctx->stream, conc ? ctx->stream2 : ctx->stream};
-  auto& evpool = ctx->lvl_events[bidx & 1];
-  size_t evn = 0;
-  // dirty[s]: stream s launched work the other stream has not waited for
-  // (the upload and the leaves are stream-0 work)
-  bool dirty[2] = {true, false};
-  auto sync_to = [&](int s) {  // stream s waits for the other stream's work so far
-    const int o = 1 - s;
-    if (!conc || !dirty[o]) return hipSuccess;
-    if (evn == evpool.size()) {
-      hipEvent_t x;
-      hipError_t r = hipEventCreateWithFlags(&x, hipEventDisableTiming);
-      if (r != hipSuccess) return r;
-      evpool.push_back(x);
-    }
-    hipEvent_t x = evpool[evn++];
-    hipError_t r = hipEventRecord(x, sk[o]);
-    if (r == hipSuccess) r = hipStreamWaitEvent(sk[s], x, 0);
-    dirty[o] = false;
-    return r;
-  };
-  auto launch = [&](int k, const char* descs, const int64_t* prefix, int nops, int64_t total) {
-    if (k == 0)
-      return launch_ew_direct(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
-                              direct_grid(ctx, total, direct_chunk(ctx, total)),
-                              direct_chunk(ctx, total), sk[0]);
-    if (k == 1)
-      return (ctx->tune.ew_prefetch ? launch_ew_pf : launch_ew)(
-          dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
-                       ew_grid(ctx, dtype, total, ew_chunk(ctx, total)), ew_chunk(ctx, total),
-                       sk[0]);
-    return launch_gemm(dtype, true, k - 2, reinterpret_cast<const GemmDesc*>(descs), prefix, nops,
-                       total, gemm_grid(ctx, total, dtype, true, k - 2), sk[1]);
-  };
-  for (int L = 1; L <= H && e == hipSuccess; L++) {
-    if (conc) {
-      // a stream launching work of this level first waits for the other
-      // stream's work of earlier levels (both syncs before any launch)
-      bool has[2] = {false, false};
-      for (int k = 0; k < kKinds; k++)
-        if (cnt[slot(k, L)]) has[k >= 2 ? 1 : 0] = true;
-      if (has[0] && e == hipSuccess) e = sync_to(0);
-      if (has[1] && e == hipSuccess) e = sync_to(1);
-      if (has[0]) dirty[0] = true;
-      if (has[1]) dirty[1] = true;
-    }
-    for (int k = 0; k < kKinds && e == hipSuccess; k++) {
-      const int w = slot(k, L);
-      if (!cnt[w]) continue;
-      const int64_t total = pfx[w].back();
-      if (per_step) {
-        for (int32_t q = 0; q < cnt[w] && e == hipSuccess; q++) {
-          char tag_s[160];
-          if (k >= 2) {
-            const GemmDesc& d = reinterpret_cast<const GemmDesc*>(hb + o_desc[w])[q];
-            std::snprintf(tag_s, sizeof tag_s, "M=%d N=%d K=%d batch=%d", d.M, d.N, d.K, d.batch);
-          } else {
-            const EwDesc& d = reinterpret_cast<const EwDesc*>(hb + o_desc[w])[q];
-            std::snprintf(tag_s, sizeof tag_s, "nT=%d nO=%d nR=%d nRo=%d nSA=%d nSB=%d op=%d",
-                          d.nT, d.nO, d.nR, d.nRo, d.nSA, d.nSB, d.mlay);
-          }
-          const int64_t tiles = pfx[w][q + 1] - pfx[w][q];
-          ev = ctx->prof.begin(ctx->stream);
-          e = launch(k, dbase + o_desc[w] + q * desc_size(k),
-                     reinterpret_cast<const int64_t*>(dbase + o_one[w]) + 2 * q, 1, tiles);
-          ctx->prof.end(ctx->stream, ev, stat_kind(k), L, 1, step_work[w][q].first,
-                        step_work[w][q].second, tag_s);
-          ctx->stats.launches++;
-        }
-        continue;
-      }
-      if (prof) ev = ctx->prof.begin(ctx->stream);
-      e = launch(k, dbase + o_desc[w], reinterpret_cast<const int64_t*>(dbase + o_pfx[w]), cnt[w],
-                 total);
-      if (prof) ctx->prof.end(ctx->stream, ev, stat_kind(k), L, cnt[w], lvl_bytes[w], lvl_macs[w]);
-      ctx->stats.launches++;
-    }
-    ctx->stats.waves++;
-  }
-  if (e == hipSuccess && conc) e = sync_to(0);  // the last GEMMs before the epilogue
-  if (e == hipSuccess) {
-    e = launch_epilogue(dtype, (const EpiDesc*)(dbase + o_epi), (int64_t)nb, dbase + o_res,
-                        ctx->stream);
-    ctx->stats.launches++;
-  }
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(ctx->results_pin[bidx & 1].host, dbase + o_res, 4 * nb, hipMemcpyDeviceToHost,
-                       ctx->stream);
-  hipEvent_t done = evs_[bidx & 1];
-  if (e == hipSuccess && done) e = hipEventRecord(done, ctx->stream);
-  ctx->stats.host_ms[2] += ms_since(t_issue);
+  const hipEvent_t up = ctx->up_events[bidx & 1];
+  hipError_t e = hipMemcpyAsync(dbase, hb, bytes, hipMemcpyHostToDevice, ctx->copy_stream);
+  if (e == hipSuccess) e = hipEventRecord(up, ctx->copy_stream);
   if (e != hipSuccess) {
-    sync_all();
+    (void)hipStreamSynchronize(ctx->copy_stream);
     ctx->pool.free(doff);
-    set_error(std::string("batched launch: ") + hipGetErrorString(e));
+    set_error(std::string("descriptor upload: ") + hipGetErrorString(e));
     return TBH_EDEVICE;
   }
+  // every launch of the previous batch is issued before this batch's first
+  if ((rc = join_issue())) {
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    ctx->pool.free(doff);
+    return rc;
+  }
+  auto job = std::make_unique<IssueJob>();
+  job->dtype = dtype;
+  job->prof = prof;
+  job->per_step = per_step;
+  job->bidx = bidx;
+  job->H = H;
+  job->nb = nb;
+  job->nleaves = nleaves;
+  job->o_leaves = o_leaves;
+  job->o_epi = o_epi;
+  job->o_res = o_res;
+  job->dbase = dbase;
+  job->hb = hb;
+  job->up = up;
+  job->done = evs_[bidx & 1];
+  job->cnt = std::move(cnt);
+  job->pfx = std::move(pfx);
+  job->o_desc = std::move(o_desc);
+  job->o_pfx = std::move(o_pfx);
+  job->o_one = std::move(o_one);
+  job->lvl_bytes = std::move(lvl_bytes);
+  job->lvl_macs = std::move(lvl_macs);
+  job->step_work = std::move(step_work);
+  issue_err_ = hipSuccess;
+  issue_msg_.clear();
+  try {
+    issuer_ = std::thread([this, ctx, j = std::move(job)] {
+      const hipError_t r = issue_batch(ctx, *j);
+      if (r != hipSuccess) {
+        issue_err_ = r;
+        issue_msg_ = std::string("batched launch: ") + hipGetErrorString(r);
+      }
+    });
+  } catch (const std::system_error&) {
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    ctx->pool.free(doff);
+    set_error("could not start the launch thread");
+    return TBH_ENOMEM;
+  }
   // this batch runs while the previous one is waited for
-  if ((rc = finish(done_tag))) {
+  void* tag_prev = nullptr;
+  if ((rc = finish(&tag_prev))) {
+    (void)join_issue();
     sync_all();
     ctx->pool.free(doff);
     return rc;
   }
+  if (done_tag) *done_tag = early_tag ? early_tag : tag_prev;
   prev_.on = true;
-  prev_.done = done;
+  prev_.done = evs_[bidx & 1];
   prev_.doff = doff;
   prev_.parity = bidx & 1;
   prev_.nb = nb;
@@ -1234,17 +1348,25 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
               std::to_string(fr) + ")");
     return TBH_ENOMEM;
   }
-  HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-  if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) {
-    (void)hipStreamDestroy(ctx->stream);
-    set_error("hipStreamCreateWithFlags failed");
+  auto release = [&] {
+    for (hipEvent_t& x : ctx->up_events)
+      if (x) (void)hipEventDestroy(x);
+    for (hipStream_t s : {ctx->stream, ctx->stream2, ctx->copy_stream})
+      if (s) (void)hipStreamDestroy(s);
+  };
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->up_events[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->up_events[1], hipEventDisableTiming) != hipSuccess) {
+    release();
+    set_error("stream or event creation failed");
     return TBH_EDEVICE;
   }
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, pool_bytes);
   if (e != hipSuccess) {
-    (void)hipStreamDestroy(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream2);
+    release();
     set_error(std::string("hipMalloc pool: ") + hipGetErrorString(e));
     return TBH_ENOMEM;
   }
@@ -1269,11 +1391,15 @@ int tbh_destroy(tbh_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
   if (ctx->base) (void)hipFree(ctx->base);
   for (auto& v : ctx->lvl_events)
     for (hipEvent_t ev : v) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : ctx->up_events)
+    if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   delete ctx;
   return 0;
 }

@@ -141,6 +141,8 @@ struct tbh_ctx {
                                // from the occupancy query in tbh_create
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // GEMM launches beside the EW launches of a level
+  hipStream_t copy_stream = nullptr;   // batched descriptor uploads, beside the batch in flight
+  hipEvent_t up_events[2] = {nullptr, nullptr};  // upload done, per batch parity
   std::vector<hipEvent_t> lvl_events[2];  // cross-stream level barriers, per batch parity
   char* base = nullptr;  // device pool
   tbh::Pool pool;
@@ -205,6 +207,8 @@ size_t region_bytes(size_t largest);
 // batch being enqueued and of the one in flight are lost; close() (also the
 // destructor) waits for the streams and releases the region.  The caller
 // holds ctx->mu from open() to close().
+struct IssueJob;  // one batch's launch sequence (tbh_runtime.cpp)
+
 class BatchRunner {
  public:
   explicit BatchRunner(tbh_ctx* ctx);
@@ -220,6 +224,7 @@ class BatchRunner {
 
  private:
   int finish(void** done_tag);
+  int join_issue();
   void sync_all();
   struct Pending {
     bool on = false;
@@ -237,6 +242,12 @@ class BatchRunner {
   int bidx_ = 0;
   hipEvent_t evs_[2] = {nullptr, nullptr};
   Pending prev_;
+  // the newest batch's launches are issued by their own thread, so that the
+  // caller packs and uploads the next batch meanwhile (a launch call blocks
+  // while the device's queue is deep)
+  std::thread issuer_;
+  hipError_t issue_err_ = hipSuccess;
+  std::string issue_msg_;
 };
 
 // Executes the plans of `slices` (plan_of(i) may block until slice i is

@@ -573,6 +573,23 @@ int64_t op_tiles(const POp& op) {
 size_t desc_size(int kind) { return kind >= 2 ? sizeof(GemmDesc) : sizeof(EwDesc); }
 int stat_kind(int kind) { return kind < 2 ? kind : 2; }  // tbh_stats: direct, tiled, GEMM
 
+}  // namespace
+
+size_t desc_bytes(const SlicePlan& P) {
+  if (P.empty) return sizeof(EpiDesc) + 4;
+  // a step: its descriptor, its prefix entry and (per-step profiling) its
+  // one-step prefix pair
+  return P.ops.size() * (std::max(sizeof(EwDesc), sizeof(GemmDesc)) + 3 * sizeof(int64_t)) +
+         P.leaves.size() * sizeof(LeafDesc) + sizeof(EpiDesc) + 4;
+}
+
+size_t desc_batch_overhead(int levels) {
+  const size_t slots = (size_t)kKinds * (size_t)(levels + 1);
+  return 256 * (3 + 3 * slots) + sizeof(int64_t) * slots;
+}
+
+namespace {
+
 // Runs f(i, t) for i in [0, n) on the caller (slot 0) and up to threads - 1
 // pool threads (slots 1..), or on the caller alone for small n; pool threads
 // busy elsewhere never hold it up.
@@ -628,6 +645,7 @@ int BatchRunner::open() {
     return TBH_ENOMEM;
   }
   arena_base_ = (uint64_t)(ctx_->base + region_off_);
+  desc_budget_ = ctx_->pool.largest_free() / 2;
   for (auto& ev : evs_)
     if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
   return 0;
@@ -649,6 +667,7 @@ void BatchRunner::close() {
   if (region_off_ >= 0) ctx_->pool.free(region_off_);
   region_off_ = -1;
   region_ = 0;
+  desc_budget_ = 0;
   if (ctx_->profiling) ctx_->prof.collect(ctx_->stats);
 }
 
@@ -727,7 +746,16 @@ hipError_t issue_batch(tbh_ctx* ctx, IssueJob& j) {
   const auto& cnt = j.cnt;
   const auto& pfx = j.pfx;
   char* dbase = j.dbase;
+  // a launch failure forced from the host (error-path tests), checked
+  // before every launch of the batch; nothing is launched in its place
+  auto injected = [ctx] {
+    int64_t v = ctx->inject_launch_failure.load();
+    while (v >= 0)
+      if (ctx->inject_launch_failure.compare_exchange_weak(v, v - 1)) return v == 0;
+    return false;
+  };
   if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, j.up, 0);
+  if (e == hipSuccess && j.nleaves && injected()) e = hipErrorLaunchFailure;
   if (e == hipSuccess)
     e = launch_leaves(dtype, (const LeafDesc*)(dbase + j.o_leaves), j.nleaves, ctx->stream);
   ctx->stats.launches += j.nleaves ? 1 : 0;
@@ -762,6 +790,7 @@ hipError_t issue_batch(tbh_ctx* ctx, IssueJob& j) {
     return r;
   };
   auto launch = [&](int k, const char* descs, const int64_t* prefix, int nops, int64_t total) {
+    if (injected()) return hipErrorLaunchFailure;
     if (k == 0)
       return launch_ew_direct(dtype, reinterpret_cast<const EwDesc*>(descs), prefix, nops, total,
                               direct_grid(ctx, total, direct_chunk(ctx, total)),
@@ -821,6 +850,7 @@ hipError_t issue_batch(tbh_ctx* ctx, IssueJob& j) {
     ctx->stats.waves++;
   }
   if (e == hipSuccess && conc) e = sync_to(0);  // the last GEMMs before the epilogue
+  if (e == hipSuccess && injected()) e = hipErrorLaunchFailure;
   if (e == hipSuccess) {
     e = launch_epilogue(dtype, (const EpiDesc*)(dbase + j.o_epi), (int64_t)j.nb, dbase + j.o_res,
                         ctx->stream);
@@ -1062,7 +1092,7 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
   BatchRunner run(ctx);
   int status = run.open();
   if (status) return status;
-  const size_t region = run.region();
+  const size_t region = run.region(), dbudget = run.desc_budget();
   std::string err;
   // the first batch is kept to a fraction of the region so that the GPU
   // starts while the later slices are still being planned
@@ -1074,7 +1104,8 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
     const size_t cap = bidx == 0 ? (size_t)(first_frac * (double)region) : region;
     std::vector<const SlicePlan*> plans;
     std::vector<const SliceOwned*> sl;
-    size_t i1 = i0, arena = 0;
+    size_t i1 = i0, arena = 0, desc = 0;
+    int levels = 0;
     while (i1 < n) {
       const SlicePlan* P = plan_of(i1, &err);
       if (!P) {
@@ -1082,14 +1113,22 @@ int execute_batched(tbh_ctx* ctx, int dtype, const std::vector<const SliceOwned*
         status = TBH_EINVAL;
         break;
       }
-      const size_t a = P->empty ? 0 : (size_t)P->arena_bytes;
-      if (i1 > i0 && arena + a > cap) break;
-      if (a > region) {
-        set_error("slice arena (" + std::to_string(a) + " bytes) does not fit the pool");
+      const size_t a = P->empty ? 0 : (size_t)P->arena_bytes, d = desc_bytes(*P);
+      const int lv = std::max(levels, P->nlevels);
+      // a batch is bounded by its arenas and by its descriptors
+      if (i1 > i0 &&
+          (arena + a > cap || desc + d + desc_batch_overhead(lv) > dbudget))
+        break;
+      if (a > region || d + desc_batch_overhead(lv) > dbudget) {
+        set_error(a > region ? "slice arena (" + std::to_string(a) + " bytes) does not fit the pool"
+                             : "slice descriptors (" + std::to_string(d) +
+                                   " bytes) do not fit the pool");
         status = TBH_ENOMEM;
         break;
       }
       arena += a;
+      desc += d;
+      levels = lv;
       plans.push_back(P);
       sl.push_back(slices[i1]);
       i1++;
@@ -1377,6 +1416,8 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
   if (const char* e = std::getenv("TBH_PLAN_THREADS"))
     ctx->plan_threads = std::min(64, std::max(1, std::atoi(e)));
   ctx->workers.start(2 * ctx->plan_threads);
+  if (const char* e = std::getenv("TBH_INJECT_LAUNCH_FAILURE"))
+    ctx->inject_launch_failure = std::max<int64_t>(-1, std::atoll(e));
   ctx->tune = default_tuning();
   resolve_scratch(device);  // for tbh_contract_slices_noalloc (device is current)
   *out = ctx.release();

@@ -155,6 +155,10 @@ struct tbh_ctx {
   tbh_tuning tune{};   // planner / launch parameters (tbh_set_tuning, tbh_autotune)
   bool profiling = false;
   tbh::Profiler prof;
+  // error-path tests: after this many batched launches the next one reports
+  // a launch failure instead of launching, once (-1: never;
+  // TBH_INJECT_LAUNCH_FAILURE at tbh_create)
+  std::atomic<int64_t> inject_launch_failure{-1};
   std::mutex serve_mu;   // guards `serve` (never held across an execution)
   std::shared_ptr<tbh::ServeState> serve;  // created by tbh_serve_start
 };
@@ -216,6 +220,9 @@ class BatchRunner {
   int open();
   void close();
   size_t region() const { return region_; }
+  // device bytes a batch's descriptors may take: half of what the pool
+  // keeps beside the region, so that two batches' descriptors fit at once
+  size_t desc_budget() const { return desc_budget_; }
   bool busy() const { return prev_.on; }
   int enqueue(int dtype, const std::vector<const SliceOwned*>& slices,
               const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* dst, void* tag,
@@ -238,6 +245,7 @@ class BatchRunner {
   tbh_ctx* ctx_;
   int64_t region_off_ = -1;
   size_t region_ = 0;
+  size_t desc_budget_ = 0;
   uint64_t arena_base_ = 0;
   int bidx_ = 0;
   hipEvent_t evs_[2] = {nullptr, nullptr};
@@ -249,6 +257,12 @@ class BatchRunner {
   hipError_t issue_err_ = hipSuccess;
   std::string issue_msg_;
 };
+
+// Upper bounds of a batch's descriptor upload: per slice (its steps,
+// leaves, epilogue and result) and per batch of at most `levels` levels
+// (the 256-byte alignment of every (kernel kind, level) array).
+size_t desc_bytes(const SlicePlan& P);
+size_t desc_batch_overhead(int levels);
 
 // Executes the plans of `slices` (plan_of(i) may block until slice i is
 // planned, nullptr = planning failed); results[i] (dtype bits) = root + r

@@ -84,6 +84,7 @@ struct ServeState {
   size_t max_resident = 0;
   double round_macs = 0;    // estimated work bound of a round
   size_t region_cap = 0;    // arena region of an idle pool: a slice's upper bound
+  size_t desc_cap = 0;      // descriptor bytes a round may take in an idle pool
   std::mutex m;
   std::condition_variable cv_plan, cv_sched, cv_done;
   std::deque<std::shared_ptr<Req>> to_plan;  // submitted, not planned
@@ -127,6 +128,11 @@ struct ServeState {
           err = "slice " + std::to_string(i) + ": arena of " +
                 std::to_string(arena_of(plans[i])) + " bytes exceeds the serving region (" +
                 std::to_string(region_cap) + " bytes of the pool)";
+        } else if (desc_bytes(plans[i]) + desc_batch_overhead(plans[i].nlevels) > desc_cap) {
+          rc = TBH_ENOMEM;
+          err = "slice " + std::to_string(i) + ": descriptors of " +
+                std::to_string(desc_bytes(plans[i])) + " bytes exceed the pool's share for them (" +
+                std::to_string(desc_cap) + " bytes)";
         }
       {
         std::lock_guard<std::mutex> g(m);
@@ -147,7 +153,7 @@ struct ServeState {
   // is ready.  Slices are taken in order -- latency lane first, then arrival
   // order, each request's slices in index order -- until the next one would
   // break a bound; the first slice is always taken.
-  std::shared_ptr<Round> form_round(size_t region) {
+  std::shared_ptr<Round> form_round(size_t region, size_t desc_budget) {
     auto R = std::make_shared<Round>();
     if (ready.empty()) return R;
     std::vector<std::shared_ptr<Req>> order;
@@ -157,21 +163,26 @@ struct ServeState {
       if (!r->priority) order.push_back(r);
     R->dtype = order.front()->dtype;
     const size_t cap = max_resident ? max_resident : (size_t)-1;
-    size_t bytes = 0;
+    size_t bytes = 0, desc = 0;
+    int levels = 0;
     double work = 0;
     bool full = false;
     for (auto& r : order) {
       if (r->dtype != R->dtype) continue;
       while (r->next < r->slices.size()) {
         const SlicePlan& P = r->plans[r->next];
-        const size_t a = arena_of(P);
+        const size_t a = arena_of(P), d = desc_bytes(P);
+        const int lv = std::max(levels, P.nlevels);
         const double w = P.empty ? 0.0 : P.node_macs;
         if (!R->items.empty() &&
-            (R->items.size() >= cap || bytes + a > region || work + w > round_macs)) {
+            (R->items.size() >= cap || bytes + a > region || work + w > round_macs ||
+             desc + d + desc_batch_overhead(lv) > desc_budget)) {
           full = true;
           break;
         }
         bytes += a;
+        desc += d;
+        levels = lv;
         work += w;
         R->items.push_back({r, r->next++});
         r->admitted = true;
@@ -284,7 +295,7 @@ struct ServeState {
         const std::string err = tbh_last_error();
         idle();
         std::lock_guard<std::mutex> g(m);
-        auto R = form_round((size_t)-1);
+        auto R = form_round((size_t)-1, (size_t)-1);
         for (auto& it : R->items) fail(it.first, "execution: " + err);
         cv_done.notify_all();
         continue;
@@ -292,7 +303,7 @@ struct ServeState {
       std::shared_ptr<Round> R;
       {
         std::lock_guard<std::mutex> g(m);
-        R = form_round(run.region());
+        R = form_round(run.region(), run.desc_budget());
         if (!R->items.empty()) rounds++;
       }
       if (R->items.empty()) {  // nothing new: finish the round in flight, go idle
@@ -369,6 +380,7 @@ int tbh_serve_start_ex(tbh_ctx* ctx, const tbh_serve_config* cfg) {
   s->round_macs =
       std::pow(2.0, cfg->round_log2_macs ? cfg->round_log2_macs : TBH_SERVE_ROUND_LOG2_MACS);
   s->region_cap = tbh::region_bytes(ctx->pool.capacity());
+  s->desc_cap = (ctx->pool.capacity() - s->region_cap) / 2;
   s->planner = std::thread([p = s.get()] { p->plan_loop(); });
   s->scheduler = std::thread([p = s.get()] { p->sched_loop(); });
   ctx->serve = s;

@@ -620,10 +620,10 @@ size_t region_bytes(size_t largest) {
 
 // Batches run one after another on the stream through one arena region
 // (stream order keeps a batch's kernels behind the previous batch's).  A
-// batch's launches are issued by a thread of its own (a launch call blocks
-// while the device's queue is deep), and its descriptors go up on a copy
-// stream, so the next batch is packed and uploaded while this one is still
-// being issued and run.  A batch is packed in two passes on the host:
+// batch's launches are issued by the context's launch thread (a launch call
+// blocks while the device's queue is deep), and its descriptors go up on a
+// copy stream, so the next batch is packed and uploaded while this one is
+// still being issued and run.  A batch is packed in two passes on the host:
 // pass 1 (serial, integer work only) counts the steps of every (kernel kind,
 // level), gives each slice its first index in each and builds the tile
 // prefixes; pass 2 (host threads, one slice at a time) writes every
@@ -652,9 +652,9 @@ int BatchRunner::open() {
 }
 
 void BatchRunner::close() {
-  if (issuer_.joinable()) issuer_.join();
-  issue_err_ = hipSuccess;  // a failed batch is released below
-  issue_msg_.clear();
+  const int issued = ctx_->issuer.wait();
+  ctx_->issuer.reset();  // a failed batch is released below
+  if (issued && !prev_.on) sync_all();
   if (prev_.on) {
     // an unfinished batch: wait for it without reporting (error paths)
     sync_all();
@@ -681,14 +681,7 @@ void BatchRunner::sync_all() {
 
 // Waits until every launch of the newest batch is issued; reports a failed
 // launch (the batch stays pending, for close() to wait for and release).
-int BatchRunner::join_issue() {
-  if (issuer_.joinable()) issuer_.join();
-  if (issue_err_ != hipSuccess) {
-    set_error(issue_msg_);
-    return TBH_EDEVICE;
-  }
-  return 0;
-}
+int BatchRunner::join_issue() { return ctx_->issuer.wait(); }
 
 // Waits for the batch in flight, releases its descriptors and copies its
 // results to their destination.
@@ -865,6 +858,70 @@ hipError_t issue_batch(tbh_ctx* ctx, IssueJob& j) {
 }
 
 }  // namespace
+
+Issuer::~Issuer() { stop(); }
+
+void Issuer::start(tbh_ctx* ctx) {
+  ctx_ = ctx;
+  stop_ = false;
+  th_ = std::thread([this] {
+    (void)hipSetDevice(ctx_->device);
+    for (;;) {
+      std::unique_ptr<IssueJob> j;
+      {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return stop_ || job_; });
+        if (!job_) return;  // stopping, nothing in hand
+        j = std::move(job_);
+      }
+      const hipError_t r = issue_batch(ctx_, *j);
+      {
+        std::lock_guard<std::mutex> g(m_);
+        if (r != hipSuccess && err_ == hipSuccess) {
+          err_ = r;
+          msg_ = std::string("batched launch: ") + hipGetErrorString(r);
+        }
+        pending_ = false;
+      }
+      cv_.notify_all();
+    }
+  });
+}
+
+void Issuer::stop() {
+  if (!th_.joinable()) return;
+  {
+    std::lock_guard<std::mutex> g(m_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  th_.join();
+}
+
+void Issuer::submit(std::unique_ptr<IssueJob> job) {
+  {
+    std::lock_guard<std::mutex> g(m_);
+    job_ = std::move(job);
+    pending_ = true;
+  }
+  cv_.notify_all();
+}
+
+int Issuer::wait() {
+  std::unique_lock<std::mutex> g(m_);
+  cv_.wait(g, [&] { return !pending_; });
+  if (err_ != hipSuccess) {
+    set_error(msg_);
+    return TBH_EDEVICE;
+  }
+  return 0;
+}
+
+void Issuer::reset() {
+  std::lock_guard<std::mutex> g(m_);
+  err_ = hipSuccess;
+  msg_.clear();
+}
 
 int BatchRunner::enqueue(int dtype, const std::vector<const SliceOwned*>& slices,
                          const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* dst,
@@ -1050,22 +1107,7 @@ int BatchRunner::enqueue(int dtype, const std::vector<const SliceOwned*>& slices
   job->lvl_bytes = std::move(lvl_bytes);
   job->lvl_macs = std::move(lvl_macs);
   job->step_work = std::move(step_work);
-  issue_err_ = hipSuccess;
-  issue_msg_.clear();
-  try {
-    issuer_ = std::thread([this, ctx, j = std::move(job)] {
-      const hipError_t r = issue_batch(ctx, *j);
-      if (r != hipSuccess) {
-        issue_err_ = r;
-        issue_msg_ = std::string("batched launch: ") + hipGetErrorString(r);
-      }
-    });
-  } catch (const std::system_error&) {
-    (void)hipStreamSynchronize(ctx->copy_stream);
-    ctx->pool.free(doff);
-    set_error("could not start the launch thread");
-    return TBH_ENOMEM;
-  }
+  ctx->issuer.submit(std::move(job));
   // this batch runs while the previous one is waited for
   void* tag_prev = nullptr;
   if ((rc = finish(&tag_prev))) {
@@ -1416,6 +1458,15 @@ int tbh_create(int device, size_t pool_bytes, tbh_ctx** out) {
   if (const char* e = std::getenv("TBH_PLAN_THREADS"))
     ctx->plan_threads = std::min(64, std::max(1, std::atoi(e)));
   ctx->workers.start(2 * ctx->plan_threads);
+  try {
+    ctx->issuer.start(ctx.get());
+  } catch (const std::system_error&) {
+    ctx->workers.stop();
+    (void)hipFree(ctx->base);
+    release();
+    set_error("could not start the launch thread");
+    return TBH_ENOMEM;
+  }
   if (const char* e = std::getenv("TBH_INJECT_LAUNCH_FAILURE"))
     ctx->inject_launch_failure = std::max<int64_t>(-1, std::atoll(e));
   ctx->tune = default_tuning();
@@ -1429,6 +1480,7 @@ int tbh_serve_stop(tbh_ctx* ctx);
 int tbh_destroy(tbh_ctx* ctx) {
   if (!ctx) return 0;
   tbh_serve_stop(ctx);
+  ctx->issuer.stop();
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);

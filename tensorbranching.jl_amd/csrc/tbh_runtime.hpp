@@ -67,6 +67,32 @@ struct Staging {  // pinned host buffer reused for descriptor uploads
 };
 
 struct ServeState;
+struct IssueJob;  // one batch's launch sequence (tbh_runtime.cpp)
+
+// A context's launch thread.  The thread that packs a batch hands its launch
+// sequence over and goes on to pack the next one (a launch call blocks while
+// the device's queue is deep); one sequence is in hand at a time.
+class Issuer {
+ public:
+  ~Issuer();
+  void start(tbh_ctx* ctx);
+  void stop();
+  void submit(std::unique_ptr<IssueJob> job);
+  // Waits until the sequence in hand, if any, is issued; 0, or TBH_EDEVICE
+  // (error set) when a launch of it failed -- until reset().
+  int wait();
+  void reset();
+
+ private:
+  tbh_ctx* ctx_ = nullptr;
+  std::thread th_;
+  std::mutex m_;
+  std::condition_variable cv_;
+  std::unique_ptr<IssueJob> job_;
+  bool pending_ = false, stop_ = false;
+  hipError_t err_ = hipSuccess;
+  std::string msg_;
+};
 
 // Host threads that live as long as the context: slice planning, descriptor
 // packing and the serving planner post work to them instead of starting
@@ -159,6 +185,7 @@ struct tbh_ctx {
   // a launch failure instead of launching, once (-1: never;
   // TBH_INJECT_LAUNCH_FAILURE at tbh_create)
   std::atomic<int64_t> inject_launch_failure{-1};
+  tbh::Issuer issuer;  // batched launch sequences
   std::mutex serve_mu;   // guards `serve` (never held across an execution)
   std::shared_ptr<tbh::ServeState> serve;  // created by tbh_serve_start
 };
@@ -211,8 +238,6 @@ size_t region_bytes(size_t largest);
 // batch being enqueued and of the one in flight are lost; close() (also the
 // destructor) waits for the streams and releases the region.  The caller
 // holds ctx->mu from open() to close().
-struct IssueJob;  // one batch's launch sequence (tbh_runtime.cpp)
-
 class BatchRunner {
  public:
   explicit BatchRunner(tbh_ctx* ctx);
@@ -228,6 +253,9 @@ class BatchRunner {
               const std::vector<const SlicePlan*>& plans, bool add_r, uint32_t* dst, void* tag,
               void** done_tag);
   int flush(void** done_tag);
+  // waits until every launch of the newest batch is issued (a failed launch
+  // is reported by the next enqueue or flush)
+  void wait_issued() { (void)join_issue(); }
 
  private:
   int finish(void** done_tag);
@@ -250,12 +278,6 @@ class BatchRunner {
   int bidx_ = 0;
   hipEvent_t evs_[2] = {nullptr, nullptr};
   Pending prev_;
-  // the newest batch's launches are issued by their own thread, so that the
-  // caller packs and uploads the next batch meanwhile (a launch call blocks
-  // while the device's queue is deep)
-  std::thread issuer_;
-  hipError_t issue_err_ = hipSuccess;
-  std::string issue_msg_;
 };
 
 // Upper bounds of a batch's descriptor upload: per slice (its steps,

@@ -300,6 +300,11 @@ struct ServeState {
         cv_done.notify_all();
         continue;
       }
+      // the next round is formed once the round in flight is issued (its
+      // last launches enter the device queue near its end), so that a
+      // request arriving meanwhile -- a latency-lane slice above all -- can
+      // still join it
+      if (inflight) run.wait_issued();
       std::shared_ptr<Round> R;
       {
         std::lock_guard<std::mutex> g(m);

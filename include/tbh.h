@@ -264,7 +264,9 @@ int tbh_plan_steps(const tbh_slice* slice, tbh_dtype dtype, tbh_mode mode,
  * chain_big absorbs side tensors of rank <= chain_side, <= chain_reduce
  * labels reduced per step); split_*: GEMM split-K (up to 2^split_tiles_bits
  * tiles per node, K parts >= 2^split_min_k_bits); ew_chunk / direct_chunk:
- * tiles per claim of the tiled / direct EW kernels; ew_per_cu: tiled EW
+ * tiles per claim of the tiled / direct EW kernels (a tiled claim above 4
+ * is halved, down to 4, on launches with too few tiles to give every
+ * workgroup of the grid such a claim); ew_per_cu: tiled EW
  * workgroups per CU in a launch's grid (at most what is resident);
  * direct_max_reduce: EW steps reducing more labels than this run LDS-tiled,
  * not as direct wave tiles (-1: all tiled); concurrent: 1 runs a level's

@@ -486,10 +486,14 @@ __device__ __forceinline__ void ew_compute(D d, const EwThread& e, const T* __re
 // are A-only and the rest of 8..11 B-only labels, so output m = ma + 2^MA mb
 // needs A value ma and B value mb: 2^MA + 2^(4-MA) LDS reads feed 16 maxima
 // per reduced configuration.
+// stA / stB: the addresses the sub-tensors now in LDS were staged from (this
+// step; 0: nothing valid).  A sub-tensor whose address repeats -- the next
+// tile differs only in labels the operand lacks -- is not staged again.
 template <typename T, bool HASB, int CNT, int MA = 0>
 __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
                                         const EwThread& e, T* __restrict__ lA,
-                                        T* __restrict__ lB) {
+                                        T* __restrict__ lB, uint64_t& stA,
+                                        uint64_t& stB) {
   const int tid = threadIdx.x;
   const TBH_GLOBAL T* __restrict__ A = reinterpret_cast<const TBH_GLOBAL T*>(d->a) + off.a;
   const TBH_GLOBAL T* __restrict__ B = reinterpret_cast<const TBH_GLOBAL T*>(d->b) + off.b;
@@ -513,8 +517,16 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
         cB -= sb;
       }
     }
-    ew_stage<T>(A + cA, d->nSA, e.loSA, e.hSA, e.lo4SA, e.x4A, lA);
-    if (HASB) ew_stage<T>(B + cB, d->nSB, e.loSB, e.hSB, e.lo4SB, e.x4B, lB);
+    const uint64_t pa = (uint64_t)(A + cA);
+    if (__builtin_amdgcn_readfirstlane((uint32_t)(pa != stA)))
+      ew_stage<T>(A + cA, d->nSA, e.loSA, e.hSA, e.lo4SA, e.x4A, lA);
+    stA = pa;
+    if (HASB) {
+      const uint64_t pb = (uint64_t)(B + cB);
+      if (__builtin_amdgcn_readfirstlane((uint32_t)(pb != stB)))
+        ew_stage<T>(B + cB, d->nSB, e.loSB, e.hSB, e.lo4SB, e.x4B, lB);
+      stB = pb;
+    }
 #if TBH_LDS_DMA
     // the DMA writes land in LDS when the loads retire (vmcnt), which the
     // barrier alone does not wait for
@@ -536,22 +548,23 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
 template <typename T, bool HASB>
 __device__ __forceinline__ void ew_tile_dispatch(KEw d, const EwOuter& off,
                                                  const EwThread& e, T* __restrict__ lA,
-                                                 T* __restrict__ lB) {
+                                                 T* __restrict__ lB, uint64_t& stA,
+                                                 uint64_t& stB) {
   switch (d->nT <= 8 ? 0 : d->nT - 8) {
-    case 0: ew_tile<T, HASB, 1>(d, off, e, lA, lB); break;
-    case 1: ew_tile<T, HASB, 2>(d, off, e, lA, lB); break;
-    case 2: ew_tile<T, HASB, 4>(d, off, e, lA, lB); break;
-    case 3: ew_tile<T, HASB, 8>(d, off, e, lA, lB); break;
+    case 0: ew_tile<T, HASB, 1>(d, off, e, lA, lB, stA, stB); break;
+    case 1: ew_tile<T, HASB, 2>(d, off, e, lA, lB, stA, stB); break;
+    case 2: ew_tile<T, HASB, 4>(d, off, e, lA, lB, stA, stB); break;
+    case 3: ew_tile<T, HASB, 8>(d, off, e, lA, lB, stA, stB); break;
     default:
       if constexpr (HASB) {
         switch (d->mlay) {
-          case 1: ew_tile<T, true, 16, 1>(d, off, e, lA, lB); return;
-          case 2: ew_tile<T, true, 16, 2>(d, off, e, lA, lB); return;
-          case 3: ew_tile<T, true, 16, 3>(d, off, e, lA, lB); return;
+          case 1: ew_tile<T, true, 16, 1>(d, off, e, lA, lB, stA, stB); return;
+          case 2: ew_tile<T, true, 16, 2>(d, off, e, lA, lB, stA, stB); return;
+          case 3: ew_tile<T, true, 16, 3>(d, off, e, lA, lB, stA, stB); return;
           default: break;
         }
       }
-      ew_tile<T, HASB, 16>(d, off, e, lA, lB);
+      ew_tile<T, HASB, 16>(d, off, e, lA, lB, stA, stB);
       break;
   }
 }
@@ -721,6 +734,7 @@ __global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW
   EwThread e;
   int op = 0;
   int64_t ob = 0, oe = 0;  // oe = 0: the first chunk searches
+  uint64_t stA = 0, stB = 0;  // sources of the LDS sub-tensors
   for (int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < total; c0 += (int64_t)gridDim.x * chunk) {
     const int64_t c1 = c0 + chunk < total ? c0 + chunk : total;
     // a workgroup's chunks only move forward: search from the current step
@@ -729,6 +743,7 @@ __global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW
       ob = uniform64(((const TBH_KCONST int64_t*)prefix)[op]);
       oe = uniform64(((const TBH_KCONST int64_t*)prefix)[op + 1]);
       ew_thread_setup((KEw)(ops + op), e);
+      stA = stB = 0;
     }
     EwOuter off{0, 0, 0};
     for (int64_t t = c0; t < c1; t++) {
@@ -738,6 +753,7 @@ __global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW
         ob = oe;
         oe = uniform64(((const TBH_KCONST int64_t*)prefix)[op + 1]);
         fresh = true;
+        stA = stB = 0;
         if (t < oe) ew_thread_setup((KEw)(ops + op), e);
       }
       KEw d = (KEw)(ops + op);
@@ -747,9 +763,9 @@ __global__ __launch_bounds__(kEwBlock) __attribute__((amdgpu_waves_per_eu(TBH_EW
       else
         outer_step(d, o, off);
       if (d->b != 0)
-        ew_tile_dispatch<T, true>(d, off, e, lA, lB);
+        ew_tile_dispatch<T, true>(d, off, e, lA, lB, stA, stB);
       else
-        ew_tile_dispatch<T, false>(d, off, e, lA, lB);
+        ew_tile_dispatch<T, false>(d, off, e, lA, lB, stA, stB);
     }
   }
 }

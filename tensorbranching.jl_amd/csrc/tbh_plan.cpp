@@ -322,6 +322,15 @@ struct Builder {
       if (inRi(l) || std::find(T.begin(), T.end(), l) != T.end()) SA.push_back(l);
     for (int32_t l : LB)
       if (inRi(l) || std::find(T.begin(), T.end(), l) != T.end()) SB.push_back(l);
+    // a workgroup's consecutive tiles take consecutive outer configurations,
+    // and the tiled kernel does not stage a sub-tensor again whose address
+    // repeats: the outer labels absent from the larger sub-tensor come first
+    // (lowest outer bits), so that it repeats most often
+    if (!op.direct && b >= 0 && opt.reuse_order) {
+      const bool bigA = SA.size() >= SB.size();
+      std::stable_partition(O.begin(), O.end(),
+                            [&](int32_t l) { return bigA ? !inA(l) : !inB(l); });
+    }
     std::vector<int32_t> Ro(R.begin() + Ri.size(), R.end());  // outer reduced
     EwDesc& d = op.ew;
     std::memset(&d, -1, sizeof d);

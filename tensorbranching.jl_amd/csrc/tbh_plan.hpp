@@ -104,6 +104,9 @@ struct PlanOptions {
   // GEMM tile rule (gemm_shape): 0 128 x 128 only, 1 tall-skinny tiles for
   // N <= 64, 2 also 256 x 64 up to N = 128
   int gemm_tiles = 1;
+  // tiled EW steps: outer labels absent from the larger sub-tensor first,
+  // so that consecutive tiles reuse it from LDS; false: C's label order
+  bool reuse_order = true;
 };
 
 // Returns 0 or sets *err and returns -1.

@@ -245,7 +245,7 @@ tbh_tuning default_tuning() {
     d.chain_reduce = o.chain_reduce;
     d.split_tiles_bits = o.split_tiles_bits;
     d.split_min_k_bits = o.split_min_k_bits;
-    d.ew_chunk = 4;
+    d.ew_chunk = 8;
     d.direct_chunk = 4;
     d.ew_per_cu = 4;
     d.direct_max_reduce = o.direct_max_reduce;
@@ -299,6 +299,11 @@ PlanOptions plan_options(const tbh_tuning& t, bool batched) {
   o.direct_max_reduce = t.direct_max_reduce;
   o.gemm_tiles = t.gemm_tiles;
   o.chain_cost = t.chain_cost;
+  static const bool reuse_order = [] {
+    const char* e = std::getenv("TBH_REUSE_ORDER");
+    return e ? std::atoi(e) != 0 : PlanOptions().reuse_order;
+  }();
+  o.reuse_order = reuse_order;
   return o;
 }
 
@@ -549,11 +554,21 @@ int direct_grid(tbh_ctx* ctx, int64_t total, int chunk) {
 
 // Waves (direct) and workgroups (tiled) take small chunks of the tile space
 // round-robin, which spreads expensive steps evenly; long contiguous ranges
-// measured slower (costly steps cluster in a few ranges).  Tiled claims of 4
-// tiles measured best with the 16x grid (n=300 solve: 2 -> 291 ms, 4 -> 280,
-// 8 -> 279, 16 -> 287).
+// measured slower (costly steps cluster in a few ranges).  Tiled claims of 8
+// tiles measured best with the 16x grid once a workgroup's consecutive tiles
+// reuse repeating sub-tensors from LDS (n=300 solve: 4 -> 268 ms median, 8 ->
+// 265, 16 -> 279; before that reuse 2 -> 291, 4 -> 280, 8 -> 279), but only
+// on launches large enough to give every workgroup of the grid such a claim:
+// smaller launches (serving rounds, small solves) were slower with 8 than
+// with 4, so a claim above 4 is halved, down to 4, until the launch fills the
+// grid with claims.
 int direct_chunk(tbh_ctx* ctx, int64_t) { return std::max(1, ctx->tune.direct_chunk); }
-int ew_chunk(tbh_ctx* ctx, int64_t) { return std::max(1, ctx->tune.ew_chunk); }
+int ew_chunk(tbh_ctx* ctx, int64_t total) {
+  int c = std::max(1, ctx->tune.ew_chunk);
+  const int64_t grid_cap = (int64_t)ctx->num_cus * ctx->tune.ew_per_cu * ctx->tune.ew_grid_mult;
+  while (c > 4 && total < (int64_t)c * grid_cap) c >>= 1;
+  return c;
+}
 
 }  // namespace
 

@@ -36,7 +36,7 @@ int main(int argc, char** argv) {
     in.ix = ix.data();
     in.tree = tree.empty() ? nullptr : tree.data();
     in.unit_wbits = 0x3f800000u;  // Float32 1
-    std::vector<tbh::PlanOptions> variants(8);
+    std::vector<tbh::PlanOptions> variants(9);
     variants[1].batched = false;
     variants[2].direct_max_reduce = tbh::kMaxReduce;
     variants[3].gemm_scatter_bits = -1;
@@ -45,6 +45,7 @@ int main(int argc, char** argv) {
     variants[5].gemm_tiles = 0;  // 128 x 128 GEMM tiles only
     variants[6].gemm_tiles = 2;  // 256 x 64 tiles up to N = 128
     variants[7].chain_cost = 0;  // chain rotations under the rank caps alone
+    variants[8].reuse_order = false;  // tiled EW outer labels in C's order
     for (const auto& opt : variants) {
       tbh::SlicePlan P;
       std::string err;

@@ -136,14 +136,15 @@ def test_tile_variants_bit_exact(tb, orc, backend, dtype):
     # direct steps at all, every launch on one stream (GEMMs not beside the
     # EW steps of their level), tiled EW steps prefetched into registers
     # (alone and with every EW step tiled), persistent grids, both GEMM tile
-    # rules besides the default, and chain rotations under the rank caps
-    # alone (no cost test)
+    # rules besides the default, chain rotations under the rank caps alone
+    # (no cost test), and claims of one tile (no sub-tensor reuse within a
+    # claim) and of sixteen
     t0 = backend.tuning()
     try:
         for kw in ({"direct_max_reduce": 12}, {"direct_max_reduce": -1}, {"concurrent": 0},
                    {"ew_prefetch": 1}, {"ew_prefetch": 1, "direct_max_reduce": -1},
                    {"ew_grid_mult": 1, "gemm_grid_mult": 1}, {"gemm_tiles": 0},
-                   {"gemm_tiles": 2}, {"chain_cost": 0}):
+                   {"gemm_tiles": 2}, {"chain_cost": 0}, {"ew_chunk": 1}, {"ew_chunk": 16}):
             backend.set_tuning(**dict(t0, **kw))
             assert np.array_equal(backend.contract_slices(br, dtype, "batched"), ref), kw
     finally:

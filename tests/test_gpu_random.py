@@ -4,7 +4,7 @@ Int32), slicing target (up to six below the graph's unsliced sc) and a
 tuning variant (defaults, every eligible EW step direct, none direct, one
 stream, no chain re-association, tiled EW prefetched into registers, persistent
 grids, 128 x 128 GEMM tiles only, wider tall-skinny GEMM tiles, chain rotations
-under the rank caps alone) drawn per case.  Each case's per-slice
+under the rank caps alone, tiled claims of one tile) drawn per case.  Each case's per-slice
 optima must equal the oracle's, bit for bit, in both executors.  Cases whose
 unsliced sc exceeds 28, whose slices exceed 2^28 work or 4000 in number are
 skipped (deterministically, from the seed), so the oracle stays within
@@ -53,7 +53,7 @@ def _case(tb, rng):
 
 VARIANTS = [{}, {"direct_max_reduce": 12}, {"direct_max_reduce": -1}, {"concurrent": 0},
             {"chain_big": 30}, {"ew_prefetch": 1}, {"ew_grid_mult": 1, "gemm_grid_mult": 1},
-            {"gemm_tiles": 0}, {"gemm_tiles": 2}, {"chain_cost": 0}]
+            {"gemm_tiles": 0}, {"gemm_tiles": 2}, {"chain_cost": 0}, {"ew_chunk": 1}]
 
 
 @pytest.mark.parametrize("group", range(GROUPS))

@@ -62,4 +62,4 @@ def test_planner_clean_under_sanitizers(tb, tmp_path):
     r = subprocess.run([str(exe), str(data)], capture_output=True, text=True, env=env,
                        timeout=600)
     assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-4000:])
-    assert "planned %d slice variants" % (8 * count) in r.stdout
+    assert "planned %d slice variants" % (9 * count) in r.stdout

@@ -915,7 +915,8 @@ void Issuer::stop() {
 
 void Issuer::submit(std::unique_ptr<IssueJob> job) {
   {
-    std::lock_guard<std::mutex> g(m_);
+    std::unique_lock<std::mutex> g(m_);
+    cv_.wait(g, [&] { return !pending_; });  // one sequence in hand at a time
     job_ = std::move(job);
     pending_ = true;
   }

@@ -76,9 +76,11 @@ typedef struct {
 } tbh_slice;
 
 /* Context: one GPU, one device memory pool of pool_bytes (0 = 80% of the
- * free memory) and two streams at the default priority: the main stream,
- * and a second one on which BATCHED runs a wave's GEMM launches beside its
- * EW launches (event-ordered; every call returns with both idle).  Replaces
+ * free memory), three streams at the default priority -- the main stream,
+ * a second one on which BATCHED runs a wave's GEMM launches beside its EW
+ * launches, and a copy stream for BATCHED descriptor uploads (event-ordered;
+ * every call returns with all of them idle) -- and a host thread that issues
+ * BATCHED launch sequences while the caller packs the next batch.  Replaces
  * the implicit CUDA.jl state used by solve(..., usecuda=true)
  * (src/dynamic_ob.jl:32). */
 int tbh_create(int device, size_t pool_bytes, tbh_ctx** out);
@@ -166,12 +168,14 @@ typedef enum {
 
 /* Start the planner and scheduler threads.  A request is refused alone
  * (FAILED) when one of its slices' arenas (tbh_slice_footprint) exceeds the
- * region an idle pool gives a batched run.  The scheduler runs rounds: a
- * round admits slices -- latency-lane requests first, then arrival order --
- * while their summed arenas fit that region, their summed estimated work
- * (the MACs of the slices' trees) stays within 2^round_log2_macs and at
- * most max_resident_slices (0: no cap) are in it.  Rounds are pipelined on
- * the stream (the next one is enqueued while one runs), so a new request
+ * region an idle pool gives a batched run, or its descriptors the pool's
+ * share for them.  The scheduler runs rounds: a round admits slices --
+ * latency-lane requests first, then arrival order -- while their summed
+ * arenas fit that region, their descriptors that share, their summed
+ * estimated work (the MACs of the slices' trees) stays within
+ * 2^round_log2_macs and at most max_resident_slices (0: no cap) are in it.
+ * Rounds are pipelined on the stream (the next one is enqueued while one
+ * runs, and formed once the one before it is issued), so a new request
  * waits for at most the round running and the one queued.  A round whose
  * execution fails is run again request by request, and only the requests
  * that fail alone end FAILED.  While requests are in flight the scheduler

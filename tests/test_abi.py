@@ -136,8 +136,8 @@ def test_footprint_bounds_reference_model(tb):
     small multiple of the reference's peak-memory model (src/utils.jl:197-219,
     x 4 bytes): the arena also holds every leaf tensor and the re-associated
     side tensors, and blocks are reused only within a size class.  On the 766
-    slices of the n=300 workload the ratio is 0.59-2.10 (median 1.38), on the
-    n=100 sc_target 10 slices 1.38-2.20; the bound is 3."""
+    slices of the n=300 workload the ratio is 0.59-2.05 (median 1.43), on the
+    n=100 sc_target 10 slices 1.23-2.34; the bound is 3."""
     g = tb.random_regular_graph(70, 3, seed=6)
     b = tb.SlicedBranch(tb.MISProblem(g), tb.initialize_code(g, 8), 0)
     fp = tb.slice_footprint(b)

@@ -307,5 +307,7 @@ PlanOptions plan_options(const tbh_ctx* ctx, bool batched);
 // TBH_DIRECT_CHUNK, TBH_FIRST_BATCH and the others listed where it is
 // defined, read once)
 tbh_tuning default_tuning();
+// CPUs the process may use (affinity mask and cgroup CPU quota)
+int usable_cpus();
 
 }  // namespace tbh

@@ -395,14 +395,14 @@ struct Builder {
   }
 };
 
-// A node runs as a tiled GEMM when M, N >= 64 and K >= 16, or when its
+// A node runs as a tiled GEMM when M, N >= 64 and K >= 16 (gemm_min_k_mn), or when its
 // outputs fill at least a quarter of a 128 x 128 register-blocked tile and K
 // is long enough (>= 64) to amortise the partly idle tile; smaller outputs
 // run as EW steps (a partly filled tile still computes the whole tile).
 bool gemm_shaped(const PlanOptions& opt, int na, int nb, int nk) {
   const int64_t M = (int64_t)1 << std::min(na, 30), N = (int64_t)1 << std::min(nb, 30),
                 K = (int64_t)1 << std::min(nk, 30);
-  if (M >= 64 && N >= 64 && K >= 16) return true;
+  if (M >= 64 && N >= 64 && K >= opt.gemm_min_k_mn) return true;
   return std::min<int64_t>(M, 128) * std::min<int64_t>(N, 128) >= opt.gemm_min_fill &&
          K >= opt.gemm_min_k;
 }

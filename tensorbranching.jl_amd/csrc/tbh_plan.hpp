@@ -73,9 +73,9 @@ struct SlicePlan {
 
 struct PlanOptions {
   bool batched = true;   // wave levels + layout planning; false: OMEinsum lowering
-  // GEMM-shaped nodes: M, N >= 64 (K >= 16), or min(M,128) x min(N,128) >=
-  // gemm_min_fill with K >= gemm_min_k
-  int gemm_min_fill = 4096, gemm_min_k = 64;
+  // GEMM-shaped nodes: M, N >= 64 (K >= gemm_min_k_mn), or min(M,128) x
+  // min(N,128) >= gemm_min_fill with K >= gemm_min_k
+  int gemm_min_k_mn = 16, gemm_min_fill = 4096, gemm_min_k = 64;
   // split-K of GEMM nodes (batched only): up to 2^split_tiles_bits tiles per
   // node, K parts of >= 2^split_min_k_bits, temporaries of rank <= split_max_rank
   bool split_k = true;

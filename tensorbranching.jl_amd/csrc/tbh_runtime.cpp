@@ -11,6 +11,7 @@
 #include "tbh_runtime.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -322,6 +323,16 @@ PlanOptions plan_options(const tbh_tuning& t, bool batched) {
     return e ? std::atoi(e) : PlanOptions().gemm_scatter_bits;
   }();
   o.gemm_scatter_bits = scatter_bits;
+  static const std::array<int, 3> gemm_min = [] {
+    const PlanOptions d;
+    std::array<int, 3> v = {d.gemm_min_k_mn, d.gemm_min_fill, d.gemm_min_k};
+    if (const char* e = std::getenv("TBH_GEMM_MIN"))
+      std::sscanf(e, "%d,%d,%d", &v[0], &v[1], &v[2]);
+    return v;
+  }();
+  o.gemm_min_k_mn = gemm_min[0];
+  o.gemm_min_fill = gemm_min[1];
+  o.gemm_min_k = gemm_min[2];
   o.direct_max_reduce = t.direct_max_reduce;
   o.gemm_tiles = t.gemm_tiles;
   o.chain_cost = t.chain_cost;

@@ -523,6 +523,10 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
         cB -= sb;
       }
     }
+    // the staging and the stores are issued at a raised wave priority, so
+    // that a workgroup's memory requests leave ahead of other waves' compute
+    // (tiled EW 151.0 against 151.9 ms per n=300 solve)
+    __builtin_amdgcn_s_setprio(3);
     const uint64_t pa = (uint64_t)(A + cA);
     if (__builtin_amdgcn_readfirstlane((uint32_t)(pa != stA)))
       ew_stage<T>(A + cA, d->nSA, e.loSA, e.hSA, e.lo4SA, e.x4A, lA);
@@ -533,6 +537,7 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
         ew_stage<T>(B + cB, d->nSB, e.loSB, e.hSB, e.lo4SB, e.x4B, lB);
       stB = pb;
     }
+    __builtin_amdgcn_s_setprio(0);
 #if TBH_LDS_DMA
     // the DMA writes land in LDS when the loads retire (vmcnt), which the
     // barrier alone does not wait for
@@ -543,6 +548,7 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
     __syncthreads();
   }
   // contiguous writes along C's lowest positions
+  __builtin_amdgcn_s_setprio(3);
   if (active) {
     uint32_t lc = e.loTC;
     asm volatile("" : "+v"(lc));
@@ -556,6 +562,7 @@ __device__ __forceinline__ void ew_tile(KEw d, const EwOuter& off,
 #endif
     }
   }
+  __builtin_amdgcn_s_setprio(0);
 }
 
 template <typename T, bool HASB>
